@@ -1,0 +1,248 @@
+"""Benchmark: table1 NUFFT forward+adjoint pairs/s on one MI355X.
+
+Workload (one "step" = one forward + one adjoint transform, end to end
+including rocFFT, inputs resident in HBM):
+    2-D, Nd = (256, 256), Kd = (512, 512), Jd = (6, 6), Ld = 1024,
+    golden-angle radial 402 spokes x 512 readout (M = 205,824), complex64,
+    image = complex Gaussian + 10 random ellipses (seeded).
+
+Besides the JSON line's headline value, the line carries
+  roofline      the dominant interpolation kernel's achieved algorithmic
+                HBM bandwidth (bytes per launch / mean launch time, HIP
+                events on the plan's stream) against 8 TB/s,
+  cpu_baseline  the reference's own C kernels (oracle/_ref, built from the
+                reference sources) with numpy FFTs, on a bounded sample,
+  stages        per-stage mean times, and the complex128 and 3-D runs.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the complex128 / 3-D extra measurements")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    return ap.parse_args()
+
+
+def c2_workload(precision="single"):
+    from mrrt.nufft_amd import trajectories as trj
+    Nd, Kd = (256, 256), (512, 512)
+    om = trj.golden_angle_2d(402, 512)
+    x = trj.complex_gaussian(Nd, seed=0) + trj.ellipse_phantom(Nd, 10, seed=0)
+    return dict(Nd=Nd, Kd=Kd, Jd=6, Ld=1024, om=om, x=x, precision=precision)
+
+
+def c3_workload():
+    from mrrt.nufft_amd import trajectories as trj
+    Nd, Kd = (128, 128, 128), (256, 256, 256)
+    om = trj.kooshball_3d(8192, 256, seed=0)
+    x = trj.complex_gaussian(Nd, seed=0)
+    return dict(Nd=Nd, Kd=Kd, Jd=6, Ld=512, om=om, x=x, precision="single")
+
+
+def interp_bytes(M, ndim, nk, nreps, real_size):
+    """Algorithmic HBM bytes of one table1 gather (or scatter) launch:
+    tm read once, samples written (read), grid read (written) once."""
+    return M * ndim * real_size + nreps * M * 2 * real_size + nreps * nk * 2 * real_size
+
+
+class GpuRun:
+    def __init__(self, wl):
+        import torch
+        from mrrt.nufft_amd import NufftBase
+        self.torch = torch
+        self.dev = torch.device("cuda", 0)
+        self.A = NufftBase(wl["Nd"], wl["om"], Jd=wl["Jd"], Kd=wl["Kd"], Ld=wl["Ld"], precision=wl["precision"],
+                           device=self.dev)
+        cdt = torch.complex64 if wl["precision"] == "single" else torch.complex128
+        nd = len(wl["Nd"])
+        xr = torch.from_numpy(np.ascontiguousarray(wl["x"])).to(self.dev).to(cdt)
+        # Fortran-ordered image Nd + (1,), resident on the device
+        self.x = xr.permute(*reversed(range(nd))).contiguous().permute(*reversed(range(nd))).unsqueeze(-1)
+        self.M = self.A.M
+        self.nk = int(np.prod(wl["Kd"]))
+        self.ndim = nd
+        self.real_size = 4 if wl["precision"] == "single" else 8
+        self.y = torch.empty((1, self.M), dtype=cdt, device=self.dev)
+        self.grid = torch.empty((1, self.nk), dtype=cdt, device=self.dev)
+        self.xo = torch.empty((1,) + tuple(reversed(wl["Nd"])), dtype=cdt, device=self.dev)
+        self.xo = self.xo.permute(*(tuple(range(nd, 0, -1)) + (0,)))
+        import ctypes
+        from mrrt.nufft_amd import _lib
+        self.ct = ctypes
+        self.L = _lib
+        self.lib = _lib.get_lib()
+        self.xs = _lib.i64arr(list(self.x.stride()))
+        self.xos = _lib.i64arr(list(self.xo.stride()))
+        self.p = self.A._plan
+        vp = ctypes.c_void_p
+        self.px, self.py, self.pg, self.pxo = (vp(self.x.data_ptr()), vp(self.y.data_ptr()),
+                                               vp(self.grid.data_ptr()), vp(self.xo.data_ptr()))
+
+    def pair(self):
+        chk, lib, p = self.L.check, self.lib, self.p
+        chk(lib.mi355nufft_forward(p, self.px, self.xs, self.py, 1, self.pg))
+        chk(lib.mi355nufft_adjoint(p, self.py, self.pxo, self.xos, 1, self.pg))
+
+    def stages(self):
+        lib, p = self.lib, self.p
+        return {
+            "deapod_pad": lambda: lib.mi355nufft_deapod_pad(p, self.px, self.xs, self.pg, 1),
+            "fft_forward": lambda: lib.mi355nufft_fft(p, self.pg, 1, -1),
+            "interp": lambda: lib.mi355nufft_interp(p, self.pg, self.py, 1),
+            "spread": lambda: lib.mi355nufft_spread(p, self.py, self.pg, 1),
+            "fft_backward": lambda: lib.mi355nufft_fft(p, self.pg, 1, 1),
+            "crop_deapod": lambda: lib.mi355nufft_crop_deapod(p, self.pg, self.pxo, self.xos, 1),
+        }
+
+    def time(self, fn, steps, warmup):
+        torch = self.torch
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        s = torch.cuda.Event(enable_timing=True)
+        e = torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(steps):
+            fn()
+        e.record()
+        torch.cuda.synchronize()
+        return s.elapsed_time(e) / steps
+
+    def stage_times(self, steps, warmup):
+        out = {}
+        for name, fn in self.stages().items():
+            def call(fn=fn):
+                rc = fn()
+                if rc:
+                    self.L.check(rc)
+            out[name] = self.time(call, steps, warmup)
+        return out
+
+
+def cpu_baseline(seconds):
+    """Reference C kernels (oracle/_ref) + numpy FFT on the host, C2 c64."""
+    from oracle import nufft_oracle as O
+    from oracle import ref_ctypes as R
+    wl = c2_workload("single")
+    kind = "reference" if R.available() else "port"
+    interp = (R.ref_interp_forward, R.ref_interp_adjoint) if R.available() else "numpy"
+    B = O.NufftOracle(wl["Nd"], wl["om"], Jd=6, Kd=wl["Kd"], Ld=1024, precision="single", interp=interp)
+    x = wl["x"].astype(np.complex64)
+    y = B.fft(x)
+    B.adj(y)
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        y = B.fft(x)
+        B.adj(y)
+        n += 1
+        if time.perf_counter() - t0 > seconds:
+            break
+    dt = (time.perf_counter() - t0) / n
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(1.0 / dt, 4), "unit": "pairs/s", "cores": threads, "kind": kind,
+            "sample": "%d forward+adjoint pairs of the same 2-D 256^2 workload (complex64); reference C "
+                      "table1 kernels (OpenMP forward, 1-thread-per-coil adjoint) + numpy FFT" % n}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every run here uses a single MI355X "
+              "(multi-GPU sharding is described in DESIGN.md, not measured)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        sys.exit(1)
+
+    wl = c2_workload("single")
+    run = GpuRun(wl)
+    print("bench: plan ready (M=%d); timing %d steps after %d warmup" % (run.M, args.steps, args.warmup),
+          file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        run.pair()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(args.steps):
+        run.pair()
+    e.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = s.elapsed_time(e) / args.steps
+    ms_wall = 1000 * wall / args.steps
+    ms = max(ms_per_step, ms_wall)
+    stages = run.stage_times(max(10, args.steps // 2), 3)
+    print("bench: stages (ms) %s" % json.dumps({k: round(v, 5) for k, v in stages.items()}),
+          file=sys.stderr, flush=True)
+    b = interp_bytes(run.M, 2, run.nk, 1, 4)
+    dom = "interp" if stages["interp"] >= stages["spread"] else "spread"
+    achieved = b / (stages[dom] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
+                "bytes_per_launch": b}
+    extra = {}
+    if not args.no_extra:
+        r2 = GpuRun(c2_workload("double"))
+        t2 = r2.time(r2.pair, max(5, args.steps // 2), 3)
+        extra["c2_complex128_pairs_per_s"] = round(1000.0 / t2, 2)
+        del r2
+        r3 = GpuRun(c3_workload())
+        t3 = r3.time(r3.pair, 5, 1)
+        st3 = r3.stage_times(5, 1)
+        extra["c3_3d_complex64_pairs_per_s"] = round(1000.0 / t3, 3)
+        extra["c3_stages_ms"] = {k: round(v, 4) for k, v in st3.items()}
+        b3 = interp_bytes(r3.M, 3, r3.nk, 1, 4)
+        extra["c3_interp_GBps"] = round(b3 / (st3["interp"] * 1e-3) / 1e9, 1)
+        extra["c3_spread_GBps"] = round(b3 / (st3["spread"] * 1e-3) / 1e9, 1)
+        del r3
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("bench: cpu baseline (~%.0f s)" % args.cpu_seconds, file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.cpu_seconds)
+    value = 1000.0 / ms
+    line = {
+        "metric": "NUFFT fwd+adj pairs/s (2D 256^2, Kd 512^2, J6, L1024, 205,824 golden-angle samples)",
+        "value": round(value, 3),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {"workload": "2D NUFFT table1 Nd=(256,256) Kd=(512,512) Jd=6 Ld=1024 golden-angle 402x512 "
+                               "complex64, forward+adjoint incl. rocFFT", "M": run.M, "nreps": 1},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "stages_ms": {k: round(v, 5) for k, v in stages.items()},
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,197 @@
+/*
+ * mi355nufft.h -- C ABI of the MI355X (gfx950) table1 NUFFT.
+ *
+ * Drop-in boundary for the table-mode ("table1") forward/adjoint NUFFT of
+ * mrrt.nufft.  Reference paths are relative to the reference tree.
+ *
+ * Conventions shared by every entry point
+ *   - return value: 0 on success, a positive MI355NUFFT_E* code otherwise;
+ *     mi355nufft_last_error() gives a message for the calling thread.
+ *   - every data pointer is a DEVICE pointer owned by the caller, except the
+ *     set-up arrays of mi355nufft_plan_create (host pointers, copied).
+ *   - complex data is interleaved (re, im), complex64 for precision 0 and
+ *     complex128 for precision 1; real data is float32 / float64.
+ *   - grids are Fortran ordered: index k1 + K1*k2 + K1*K2*k3; a batch of
+ *     nreps grids (coils / repetitions) is stored rep-major with distance
+ *     prod(Kd).  Samples are stored rep-major: y[r*M + m].
+ *   - all work is enqueued on the plan's HIP stream (asynchronous).  A plan
+ *     serves one stream at a time; distinct plans may run concurrently.
+ *   - sizes are int64 (the reference's int32 index products overflow beyond
+ *     2^31 grid elements, nufft_table.template.c:407).
+ */
+#ifndef MI355NUFFT_H
+#define MI355NUFFT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MI355NUFFT_ABI_VERSION 1
+
+enum {
+    MI355NUFFT_OK = 0,
+    MI355NUFFT_EINVAL = 1,  /* invalid argument (sizes, null pointers, dtype) */
+    MI355NUFFT_EHIP = 2,    /* HIP runtime / kernel launch error */
+    MI355NUFFT_EFFT = 3,    /* rocFFT error */
+    MI355NUFFT_ENOMEM = 4,  /* device allocation failed */
+    MI355NUFFT_ESTATE = 5   /* call out of order (e.g. no points set) */
+};
+
+typedef struct mi355nufft_plan mi355nufft_plan;
+
+/* Message for the last failing call on this thread ("" if none). */
+const char *mi355nufft_last_error(void);
+
+/* MI355NUFFT_ABI_VERSION of the loaded library. */
+int mi355nufft_abi_version(void);
+
+/*
+ * Plan creation.  Replaces the device-side state built by the table branch of
+ * NufftBase.__init__ (mrrt/nufft/_nufft.py:216-359, tables :880-935 and
+ * :1195-1243, sn :727-748, phase_before :703-715) and _init_gpu (:362-390).
+ *   ndim            1, 2 or 3
+ *   Nd, Kd, Jd      image size, oversampled grid size, kernel taps per axis
+ *   L               table oversampling (table length J_d*L + 1)
+ *   precision       0: complex64 / float32, 1: complex128 / float64
+ *   phasing         0: real kernel + phase_before/phase_after ('real'),
+ *                   1: complex kernel ('complex')
+ *   ortho           nonzero: FFT normalised by 1/sqrt(prod(Kd)) each way
+ *   adjoint_scale   adjoint_scalefactor (_nufft.py:1564-1568)
+ *   tables          host: ndim arrays of J_d*L+1 table values, real T for
+ *                   phasing 0 or interleaved complex T for phasing 1
+ *   sn_factors      host: ndim float64 arrays of N_d deapodization factors;
+ *                   sn[n] = (T)(sn_0[n0] * sn_1[n1] * sn_2[n2]) (double product)
+ *   phase_coef      host: ndim float64 values (2*pi/K_d)*n_mid_d used by
+ *                   phase_before; NULL for phasing 1
+ *   stream          hipStream_t used for all work of this plan
+ */
+int mi355nufft_plan_create(mi355nufft_plan **plan, int ndim, const int64_t *Nd,
+                           const int64_t *Kd, const int32_t *Jd, int32_t L,
+                           int precision, int phasing, int ortho,
+                           double adjoint_scale, const void *const *tables,
+                           const double *const *sn_factors,
+                           const double *phase_coef, void *stream);
+
+/* Release every device resource of the plan (waits for its stream). */
+int mi355nufft_plan_destroy(mi355nufft_plan *plan);
+
+/* Change the stream used by subsequent calls on this plan. */
+int mi355nufft_set_stream(mi355nufft_plan *plan, void *stream);
+
+/*
+ * Non-uniform points.  Replaces tm = omega / (2 pi / K_d) (_nufft.py:338-342,
+ * evaluated after the cast of omega to the real dtype at :315), phase_after
+ * (_nufft.py:717-724, from the uncast omega at :313) and phase_shift of the
+ * complex kernel (_nufft.py:898-901), and sorts the points into bins of
+ * the oversampled grid for the tiled kernels.
+ *   omega        device (M, ndim) real array, element (m, d) at
+ *                omega[m*stride_m + d*stride_d]
+ *   omega_double 0: omega is float32, 1: float64
+ *   shift        ndim float64 phase shifts: (n_shift - n_mid) rounded to the
+ *                real dtype (phase_mode 1) or n_shift (phase_mode 2)
+ *   phase_mode   0: no per-sample phase; 1: phase_after of the real kernel;
+ *                2: phase_shift of the complex kernel
+ *   dot_single   phase_mode 1 only: nonzero evaluates omega . shift in
+ *                float32 (float32 omega and precision 0), else in float64
+ */
+int mi355nufft_set_points(mi355nufft_plan *plan, const void *omega,
+                          int omega_double, int64_t M, int64_t stride_m,
+                          int64_t stride_d, const double *shift, int phase_mode,
+                          int dot_single);
+
+/* Number of points set by the last mi355nufft_set_points (0 if none). */
+int64_t mi355nufft_num_points(const mi355nufft_plan *plan);
+
+/*
+ * Forward transform, image -> non-uniform samples.  Replaces nufft_forward
+ * (mrrt/nufft/_nufft.py:1275-1397) in table mode, including NufftBase.fft's
+ * repetitions (:425-452).
+ *   x          device complex image batch; element (n0, n1, n2, r) at
+ *              x[n0*xs[0] + n1*xs[1] + n2*xs[2] + r*xs[3]] (elements;
+ *              unused axes ignored)
+ *   y          device complex (nreps, M), written
+ *   grid       device scratch of nreps*prod(Kd) complex, or NULL to use a
+ *              plan-owned buffer
+ */
+int mi355nufft_forward(mi355nufft_plan *plan, const void *x, const int64_t *xs,
+                       void *y, int32_t nreps, void *grid);
+
+/*
+ * Adjoint transform, non-uniform samples -> image.  Replaces nufft_adj
+ * (mrrt/nufft/_nufft.py:1459-1578) in table mode.
+ *   y          device complex (nreps, M)
+ *   x          device complex image batch with element strides xs, written
+ */
+int mi355nufft_adjoint(mi355nufft_plan *plan, const void *y, void *x,
+                       const int64_t *xs, int32_t nreps, void *grid);
+
+/* ---- stages (each replaces one step of nufft_forward / nufft_adj) ---- */
+
+/* x * sn zero-padded onto the Kd grid: _nufft.py:1326 and the zero-pad of
+ * fftn(xk, Kd) at :1331. */
+int mi355nufft_deapod_pad(mi355nufft_plan *plan, const void *x,
+                          const int64_t *xs, void *grid, int32_t nreps);
+
+/* In-place C2C FFT of nreps Kd grids (rocFFT); direction -1 forward
+ * (unnormalised, :1331), +1 backward (unnormalised; the reference's ifftn
+ * 1/prod(Kd) is folded into crop_deapod, :1524 and :1564-1568). */
+int mi355nufft_fft(mi355nufft_plan *plan, void *grid, int32_t nreps,
+                   int direction);
+
+/* Forward interpolation: grid * phase_before (/ sqrt(prod Kd) if ortho),
+ * table1 gather, * phase_after / phase_shift: _nufft.py:1370-1391 with
+ * _nufft_table_interp (:998-1097).  The grid is left unchanged. */
+int mi355nufft_interp(mi355nufft_plan *plan, const void *grid, void *y,
+                      int32_t nreps);
+
+/* Adjoint gridding: y * conj(phase_after), table1 scatter onto a zeroed
+ * grid, * conj(phase_before): _nufft.py:1495-1520 with _nufft_table_adj
+ * (:1101-1191). */
+int mi355nufft_spread(mi355nufft_plan *plan, const void *y, void *grid,
+                      int32_t nreps);
+
+/* Crop [:N] of the backward-FFT grid, scale, * conj(sn): _nufft.py:1561-1572. */
+int mi355nufft_crop_deapod(mi355nufft_plan *plan, const void *grid, void *x,
+                           const int64_t *xs, int32_t nreps);
+
+/* ---- raw table1 kernels (no plan) ---- */
+
+/*
+ * Table1 gather on device buffers.  Replaces the 12 C kernels
+ * {float,double}_interp{1,2,3}_table1_{real,complex}_forward
+ * (mrrt/nufft/_extensions/c/nufft_table.template.h:34-278, bodies in
+ * nufft_table.template.c:43-141, 538-703, 710-921), the Cython bridge
+ * _interp{1,2,3}_table_forward (_extensions/_nufft_table.pyx:29-117,
+ * 215-325, 444-571) and the CUDA kern_forward launch (_nufft.py:1057-1084).
+ *   ck      device complex grids (nreps, prod(Kd))
+ *   h       host array of ndim DEVICE table pointers (J_d*L+1 entries each;
+ *           real T, or interleaved complex T if complex_kernel)
+ *   tm      device real (ndim, M): tm[d*M + m]  (the reference's p_tm)
+ *   fm      device complex (nreps, M), written
+ */
+int mi355nufft_table1_forward(int ndim, const int64_t *Kd, const int32_t *Jd,
+                              int32_t L, int precision, int complex_kernel,
+                              const void *ck, const void *const *h,
+                              const void *tm, int64_t M, void *fm,
+                              int32_t nreps, void *stream);
+
+/*
+ * Table1 scatter ck[k] += conj(coef) * fm[m] onto zeroed grids.  Replaces
+ * {float,double}_interp{1,2,3}_table1_{real,complex}_adj (nufft_table.template.c:
+ * 144-305, 308-531, 924-1202), _interp{1,2,3}_table_adj (_nufft_table.pyx:
+ * 120-212, 328-441, 574-704) and the CUDA kern_adj launch (_nufft.py:1166-1189).
+ * The output grids are zeroed first, as the C kernels do (template.c:336-337).
+ */
+int mi355nufft_table1_adjoint(int ndim, const int64_t *Kd, const int32_t *Jd,
+                              int32_t L, int precision, int complex_kernel,
+                              const void *fm, const void *const *h,
+                              const void *tm, int64_t M, void *ck,
+                              int32_t nreps, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MI355NUFFT_H */

@@ -1,0 +1,405 @@
+"""NufftBase drop-in: the table1 NUFFT on MI355X.
+
+Keeps the constructor and call surface of the reference's ``NufftBase``
+(mrrt/nufft/_nufft.py:216-482): ``NufftBase(Nd, omega, Jd, Kd, precision,
+mode, Ld, ortho, n_shift, phasing, adjoint_scalefactor, preplan_cufft, order,
+verbose, on_gpu)`` with ``.fft(x)`` and ``.adj(k)``, Fortran-ordered
+semantics (image ``Nd + (nreps,)``, samples ``(M, nreps)``) and the
+``order='C'`` repetition swap (:415-423).
+
+All transform work runs in ``libmi355nufft.so`` (HIP kernels + rocFFT) on the
+current torch stream; torch tensors serve as device buffers.  Inputs may be
+numpy arrays (copied to the device, results returned as numpy, like the
+reference's CPU path) or torch tensors (results stay on the device).
+"""
+
+import ctypes
+import warnings
+from math import sqrt
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._kernel_math import beatty_alpha, build_table, deapod_factors
+
+__all__ = ["NufftBase", "nufft_forward", "nufft_adj"]
+
+_SUPPORTED_REAL = (np.float32, np.float64)
+
+
+def _as_tuple(seq, type=int, n=None):
+    """_utils.py:130-137."""
+    if np.isscalar(seq):
+        if n is None:
+            raise ValueError("for scalar, seq, n must be specified")
+        return (type(seq),) * n
+    elif n is not None and len(seq) != n:
+        raise ValueError("array did not have the expected size of {}".format(n))
+    return tuple(type(s) for s in seq)
+
+
+def _default_device(device):
+    if device is not None:
+        return torch.device(device)
+    if not torch.cuda.is_available():
+        raise RuntimeError("mrrt.nufft_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _f_reshape(t, shape):
+    """numpy ``reshape(shape, order='F')`` for a torch tensor."""
+    shape = tuple(shape)
+    if tuple(t.shape) == shape:
+        return t
+    if tuple(t.shape) + (1,) == shape:
+        return t.unsqueeze(-1)
+    flat = t.permute(*reversed(range(t.dim()))).reshape(-1)
+    return flat.reshape(tuple(reversed(shape))).permute(*reversed(range(len(shape))))
+
+
+class NufftBase(object):
+    """Table-mode ("table1") NUFFT operator on one MI355X.
+
+    Parameters follow the reference (mrrt/nufft/_nufft.py:122-233).  ``om``
+    is accepted as an alias of ``omega``.  ``mode`` must contain "table"
+    (``'table'`` or ``'table1'``); sparse-matrix mode is not provided.
+    ``device`` selects the GPU (default: the current torch device).
+    """
+
+    def __init__(self, Nd, omega=None, Jd=4, Kd=None, precision="single", mode="table", Ld=1024,
+                 ortho=False, n_shift=None, phasing="real", adjoint_scalefactor=1.0,
+                 preplan_cufft=True, order="F", verbose=False, on_gpu=True, om=None, device=None):
+        if omega is None:
+            omega = om
+        if omega is None:
+            raise ValueError("omega is required")
+        self.verbose = verbose
+        self.order = order
+        if order not in ("F", "C"):
+            raise ValueError("order must be 'F' or 'C'")
+        self.on_gpu = True
+        self.device = _default_device(device)
+        if np.isscalar(Nd):
+            Nd = (Nd,)
+        self._Nd = _as_tuple(Nd, type=int)
+        self.ndim = len(self._Nd)
+        if phasing not in ("real", "complex"):
+            raise ValueError("Invalid phasing: {}. phasing must be 'real' or 'complex'".format(phasing))
+        self._phasing = phasing
+        self._Jd = _as_tuple(Jd, type=int, n=self.ndim)
+        if Kd is None:
+            Kd = tuple(int(1.5 * n) for n in self._Nd)
+        self._Kd = _as_tuple(Kd, type=int, n=self.ndim)
+        if self.ndim > 3:
+            raise NotImplementedError("dimensions > 3 not implemented")
+        self.ortho = ortho
+        self.scale_ortho = sqrt(float(np.prod(self._Kd))) if ortho else 1
+        if not ("table" in mode):
+            if mode == "exact":
+                raise ValueError("mode exact not implemented")
+            if mode == "sparse":
+                raise NotImplementedError("sparse-matrix mode is not provided; use mode='table'")
+            raise ValueError("Invalid NUFFT mode: {}".format(mode))
+        self._mode = mode
+        self._Ld = int(Ld) if Ld is not None else 2 ** 10
+        self.adjoint_scalefactor = adjoint_scalefactor
+        self.preplan_cufft = preplan_cufft
+
+        omega_np, omega_t = self._check_omega(omega)
+        if precision == "auto":
+            precision = "single" if omega_t.dtype == torch.float32 else "double"
+        if precision == "single":
+            self._cplx_dtype, self._real_dtype = np.dtype(np.complex64), np.dtype(np.float32)
+        elif precision == "double":
+            self._cplx_dtype, self._real_dtype = np.dtype(np.complex128), np.dtype(np.float64)
+        else:
+            raise ValueError("precision must be 'single', 'double' or 'auto'")
+        self._precision = precision
+        self._tcdt = torch.complex64 if precision == "single" else torch.complex128
+        self._trdt = torch.float32 if precision == "single" else torch.float64
+
+        if n_shift is None:
+            self._n_shift = (0.0,) * self.ndim
+        else:
+            self._n_shift = _as_tuple(n_shift, type=float, n=self.ndim)
+        if (self.ndim != len(self._Jd)) or (self.ndim != len(self._Kd)):
+            raise ValueError("Inconsistent Dimensions")
+        if phasing == "real":
+            self.n_mid = tuple(n // 2 for n in self._Nd)
+        else:
+            self.n_mid = tuple((n - 1) / 2.0 for n in self._Nd)
+        for d in range(self.ndim):
+            if self._Nd[d] > self._Kd[d]:
+                raise ValueError("Kd must be >= Nd on every axis")
+        odd_L = self._Ld % 2 == 1
+        if odd_L and any(j % 2 == 1 for j in self._Jd):
+            warnings.warn("accuracy may be compromised when L and J are both odd")
+
+        self.alpha = [beatty_alpha(self._Jd[d], self._Kd[d], self._Nd[d]) for d in range(self.ndim)]
+        self._sn_factors = [np.ascontiguousarray(
+            deapod_factors(self._Nd[d], self._Kd[d], self._Jd[d], self.alpha[d], self.n_mid[d]), dtype=np.float64)
+            for d in range(self.ndim)]
+        tables = []
+        for d in range(self.ndim):
+            h = build_table(self._Nd[d], self._Jd[d], self._Kd[d], self._Ld, phasing)
+            h = h.astype(self._cplx_dtype if phasing == "complex" else self._real_dtype)
+            tables.append(np.ascontiguousarray(h))
+        self._tables_host = tables
+
+        lib = _lib.get_lib()
+        self._lib = lib
+        self._plan = ctypes.c_void_p()
+        tab_ptrs = (ctypes.c_void_p * self.ndim)(*[t.ctypes.data for t in tables])
+        sn_ptrs = (ctypes.POINTER(ctypes.c_double) * self.ndim)(
+            *[s.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for s in self._sn_factors])
+        phase_coef = _lib.dblarr([2 * np.pi / self._Kd[d] * self.n_mid[d] for d in range(self.ndim)])
+        with torch.cuda.device(self.device):
+            self._stream = torch.cuda.current_stream(self.device).cuda_stream
+            _lib.check(lib.mi355nufft_plan_create(
+                ctypes.byref(self._plan), self.ndim, _lib.i64arr(self._Nd), _lib.i64arr(self._Kd),
+                _lib.i32arr(self._Jd), self._Ld, 0 if precision == "single" else 1,
+                0 if phasing == "real" else 1, 1 if ortho else 0, float(adjoint_scalefactor),
+                tab_ptrs, sn_ptrs, phase_coef if phasing == "real" else None, ctypes.c_void_p(self._stream)))
+        self._set_points(omega_np, omega_t)
+        self.nargin1 = int(np.prod(self._Nd))
+        self.nargout1 = self.M
+
+    # ------------------------------------------------------------------
+    # properties (names of the reference attributes)
+    # ------------------------------------------------------------------
+    Nd = property(lambda self: self._Nd)
+    Kd = property(lambda self: self._Kd)
+    Jd = property(lambda self: self._Jd)
+    Ld = property(lambda self: self._Ld)
+    mode = property(lambda self: self._mode)
+    phasing = property(lambda self: self._phasing)
+    precision = property(lambda self: self._precision)
+    n_shift = property(lambda self: self._n_shift)
+
+    @property
+    def omega(self):
+        """Sampling points cast to the real dtype, (M, ndim) on the device."""
+        return self._omega_rdt
+
+    @property
+    def h(self):
+        """Interpolation tables (one per axis) as device tensors."""
+        return [torch.from_numpy(t).to(self.device) for t in self._tables_host]
+
+    @property
+    def tm(self):
+        """omega / (2 pi / Kd), (M, ndim), as in _nufft.py:338-342."""
+        tm = torch.empty_like(self._omega_rdt)
+        for d in range(self.ndim):
+            gam = np.float64(2 * np.pi / self._Kd[d])
+            g = torch.tensor(gam, dtype=torch.float64).to(self._trdt).item()
+            tm[:, d] = (self._omega_rdt[:, d].double() / g).to(self._trdt)
+        return tm
+
+    @property
+    def sn(self):
+        """Deapodization factors sn[Nd] (float64 outer product cast to rdt)."""
+        s = np.array([1.0])
+        for f in self._sn_factors:
+            s = np.outer(s.ravel(), f)
+        return torch.from_numpy(s.reshape(self._Nd).astype(self._real_dtype)).to(self.device)
+
+    @property
+    def phase_before(self):
+        """exp(i (2 pi n_mid / Kd) . k) evaluated in the real dtype (_nufft.py:703-715)."""
+        if self._phasing != "real":
+            return None
+        rdt = self._real_dtype
+        Kd, n_mid = self._Kd, self.n_mid
+        phase = (2 * np.pi / Kd[0] * n_mid[0]) * np.arange(Kd[0], dtype=rdt)
+        for d in range(1, self.ndim):
+            tmp = (2 * np.pi / Kd[d] * n_mid[d]) * np.arange(Kd[d], dtype=rdt)
+            phase = phase.reshape(phase.shape + (1,)) + tmp.reshape((1,) * d + (tmp.size,))
+        return torch.from_numpy(np.exp(1j * phase).astype(self._cplx_dtype, copy=False)).to(self.device)
+
+    @property
+    def phase_after(self):
+        """exp(i omega . (n_shift - n_mid)) from the uncast omega (_nufft.py:717-724)."""
+        if self._phasing != "real":
+            return None
+        shift = np.asarray([s - m for s, m in zip(self._n_shift, self.n_mid)], dtype=self._real_dtype)
+        ph = np.exp(1j * np.dot(self._omega_host, shift))
+        return torch.from_numpy(ph.astype(self._cplx_dtype, copy=False)).to(self.device)
+
+    # ------------------------------------------------------------------
+    def _check_omega(self, omega):
+        """_init_omega (_nufft.py:552-573)."""
+        if isinstance(omega, torch.Tensor):
+            omega_t = omega.detach().to(self.device)
+            if omega_t.dim() == 1:
+                omega_t = omega_t[:, None]
+            omega_np = None
+        else:
+            omega_np = np.asarray(omega)
+            if omega_np.ndim == 1:
+                omega_np = omega_np[:, np.newaxis]
+            if omega_np.dtype not in _SUPPORTED_REAL:
+                raise ValueError("omega must be one of the following types: {}".format(
+                    [np.float32, np.float64]))
+            omega_t = torch.from_numpy(np.ascontiguousarray(omega_np)).to(self.device)
+        if omega_t.dim() != 2 or omega_t.shape[1] != self.ndim:
+            raise ValueError("number of cols must match NUFFT dimension")
+        if omega_t.dtype not in (torch.float32, torch.float64):
+            raise ValueError("omega must be one of the following types: {}".format([np.float32, np.float64]))
+        return omega_np, omega_t
+
+    def _set_points(self, omega_np, omega_t):
+        self.M = int(omega_t.shape[0])
+        self._omega_user = omega_t
+        self._omega_host = omega_np if omega_np is not None else omega_t.cpu().numpy()
+        self._omega_rdt = omega_t.to(self._trdt)
+        if self._phasing == "real":
+            shift = [float(np.asarray(s - m, dtype=self._real_dtype)) for s, m in zip(self._n_shift, self.n_mid)]
+            phase_mode = 1
+            dot_single = int(omega_t.dtype == torch.float32 and self._precision == "single")
+        elif any(s != 0 for s in self._n_shift):
+            shift = list(self._n_shift)
+            phase_mode = 2
+            dot_single = 0
+        else:
+            shift = [0.0] * self.ndim
+            phase_mode = 0
+            dot_single = 0
+        om = omega_t
+        self._sync_stream()
+        _lib.check(self._lib.mi355nufft_set_points(
+            self._plan, ctypes.c_void_p(om.data_ptr()), 1 if om.dtype == torch.float64 else 0, self.M,
+            om.stride(0), om.stride(1), _lib.dblarr(shift + [0.0] * (3 - self.ndim)), phase_mode, dot_single))
+
+    def _sync_stream(self):
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        if s != self._stream:
+            _lib.check(self._lib.mi355nufft_set_stream(self._plan, ctypes.c_void_p(s)))
+            self._stream = s
+
+    def __del__(self):
+        plan = getattr(self, "_plan", None)
+        lib = getattr(self, "_lib", None)
+        if plan is not None and lib is not None and plan.value:
+            try:
+                lib.mi355nufft_plan_destroy(plan)
+            except Exception:
+                pass
+            self._plan = None
+
+    # ------------------------------------------------------------------
+    def _swap_reps(self, x, narg):
+        """_nufft.py:415-418."""
+        if _numel(x) != narg:
+            x = x.permute(*(tuple(range(1, x.dim())) + (0,))) if isinstance(x, torch.Tensor) else \
+                x.transpose(tuple(range(1, x.ndim)) + (0,))
+        return x
+
+    def _unswap_reps(self, x, narg):
+        """_nufft.py:420-423."""
+        if _numel(x) != narg:
+            x = x.permute(*((x.dim() - 1,) + tuple(range(x.dim() - 1)))) if isinstance(x, torch.Tensor) else \
+                x.transpose((x.ndim - 1,) + tuple(range(x.ndim - 1)))
+        return x
+
+    def fft(self, x):
+        """Forward NUFFT (uniform spatial -> non-uniform frequency), _nufft.py:425-452."""
+        if self.order == "C":
+            x = self._swap_reps(x, self.nargin1)
+        k = nufft_forward(self, x)
+        if self.order == "C":
+            k = self._unswap_reps(k, self.nargout1)
+        return k
+
+    def adj(self, k):
+        """Adjoint NUFFT (non-uniform frequency -> uniform spatial), _nufft.py:454-482."""
+        if self.order == "C":
+            k = self._swap_reps(k, self.nargout1)
+        x = nufft_adj(self, k)
+        if self.order == "C":
+            x = self._unswap_reps(x, self.nargin1)
+        return x
+
+    # ------------------------------------------------------------------
+    # device-tensor entry points used by autograd / batching
+    # ------------------------------------------------------------------
+    def _forward_dev(self, x, nreps):
+        """x: device tensor, logical shape Nd + (nreps,) (any strides); returns (nreps, M)."""
+        y = torch.empty((nreps, self.M), dtype=self._tcdt, device=self.device)
+        grid = torch.empty((nreps, int(np.prod(self._Kd))), dtype=self._tcdt, device=self.device)
+        xs = list(x.stride())
+        self._sync_stream()
+        _lib.check(self._lib.mi355nufft_forward(
+            self._plan, ctypes.c_void_p(x.data_ptr()), _lib.i64arr(xs), ctypes.c_void_p(y.data_ptr()),
+            nreps, ctypes.c_void_p(grid.data_ptr())))
+        return y
+
+    def _adjoint_dev(self, y, nreps, out=None):
+        """y: device tensor (nreps, M) contiguous; returns Nd + (nreps,) Fortran-ordered."""
+        if out is None:
+            out = torch.empty((nreps,) + tuple(reversed(self._Nd)), dtype=self._tcdt, device=self.device)
+            out = out.permute(*(tuple(range(self.ndim, 0, -1)) + (0,)))
+        grid = torch.empty((nreps, int(np.prod(self._Kd))), dtype=self._tcdt, device=self.device)
+        self._sync_stream()
+        _lib.check(self._lib.mi355nufft_adjoint(
+            self._plan, ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+            _lib.i64arr(list(out.stride())), nreps, ctypes.c_void_p(grid.data_ptr())))
+        return out
+
+
+def _numel(x):
+    return x.numel() if isinstance(x, torch.Tensor) else np.size(x)
+
+
+def _to_device(obj, x):
+    """Returns (tensor on device with the operator's complex dtype, was_numpy)."""
+    was_numpy = not isinstance(x, torch.Tensor)
+    if was_numpy:
+        x = np.asarray(x)
+        if x.dtype == np.complex128 and obj._precision == "single":
+            x = x.astype(np.complex64)
+        t = torch.from_numpy(x) if x.dtype.kind in "fc" else torch.from_numpy(x.astype(np.float64))
+        t = t.to(obj.device)
+    else:
+        t = x.to(obj.device)
+    if not t.is_complex():
+        t = t.to(obj._trdt)
+        t = torch.complex(t, torch.zeros_like(t))
+    if t.dtype != obj._tcdt:
+        t = t.to(obj._tcdt)
+    return t, was_numpy
+
+
+def _to_output(t, was_numpy):
+    if was_numpy:
+        return t.cpu().numpy()
+    return t
+
+
+def nufft_forward(obj, x):
+    """Forward table1 NUFFT (mrrt/nufft/_nufft.py:1275-1397)."""
+    t, was_numpy = _to_device(obj, x)
+    nn = obj.nargin1
+    if t.numel() % nn != 0:
+        raise ValueError("Input signal has the wrong size.")
+    nreps = t.numel() // nn
+    xs = _f_reshape(t, tuple(obj.Nd) + (nreps,))
+    y = obj._forward_dev(xs, nreps)           # (nreps, M) = Fortran (M, nreps)
+    out = y.t()
+    if nreps == 1:
+        out = out[:, 0]
+    return _to_output(out, was_numpy)
+
+
+def nufft_adj(obj, xk):
+    """Adjoint table1 NUFFT (mrrt/nufft/_nufft.py:1459-1578)."""
+    t, was_numpy = _to_device(obj, xk)
+    if t.numel() % obj.M != 0:
+        raise ValueError("invalid size")
+    nreps = t.numel() // obj.M
+    y = _f_reshape(t, (obj.M, nreps)).t().contiguous()   # (nreps, M)
+    out = obj._adjoint_dev(y, nreps)
+    if nreps == 1:
+        out = out[..., 0]
+    return _to_output(out, was_numpy)

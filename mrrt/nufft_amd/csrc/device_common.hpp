@@ -1,0 +1,143 @@
+// Device-side helpers shared by the table1 NUFFT kernels (gfx950).
+//
+// Arithmetic that decides which table cell and which grid point a tap uses
+// follows the reference C kernels operation for operation
+// (mrrt/nufft/_extensions/c/nufft_table.template.c:577-601):
+//   k0  = 1 + floor(t - J/2.)          evaluated in double
+//   p   = (t - k) * L                  evaluated in T (float or double)
+//   n   = floor(p),  alf = p - n       T
+//   c   = (1 - alf) * h[n + nc] + alf * h[n + 1 + nc],   nc = floor(J*L/2)
+// FP contraction is disabled in this file so that every operation above is
+// rounded exactly as in the reference (no fused multiply-add).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace mi355 {
+
+template <typename T> struct CxOf;
+template <> struct CxOf<float> { using type = float2; };
+template <> struct CxOf<double> { using type = double2; };
+template <typename T> using cx = typename CxOf<T>::type;
+
+// table element: real T for the real kernel, complex T for the complex kernel
+template <typename T, bool CK> struct TabOf { using type = T; };
+template <typename T> struct TabOf<T, true> { using type = cx<T>; };
+template <typename T, bool CK> using tab = typename TabOf<T, CK>::type;
+
+template <typename T>
+__device__ __forceinline__ cx<T> mkcx(T re, T im) {
+    cx<T> c;
+    c.x = re;
+    c.y = im;
+    return c;
+}
+
+__device__ __forceinline__ float floor_t(float v) { return floorf(v); }
+__device__ __forceinline__ double floor_t(double v) { return floor(v); }
+__device__ __forceinline__ float fmin_t(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ double fmin_t(double a, double b) { return fmin(a, b); }
+__device__ __forceinline__ float fmax_t(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ double fmax_t(double a, double b) { return fmax(a, b); }
+
+// Per-axis table geometry.  The table pointer is shifted to its centre
+// (nufft_table.template.c:58-63) and the cell index n is kept in
+// [nlo, nhi] = [-nc, J*L - 1 - nc] so that h[n] and h[n+1] stay inside the
+// J*L+1 entries even for the rounding edge case p == J*L/2 (where the
+// clamped cell gives the same coefficient) and for non-finite t.
+struct AxisGeom {
+    int K;      // oversampled grid size
+    int J;      // taps
+    int nlo;    // -nc
+    int nhi;    // J*L - 1 - nc
+};
+
+// First grid index (mod K) of the J taps of coordinate t.
+template <typename T>
+__device__ __forceinline__ long long first_tap(T t, int J) {
+    double kd = floor((double)t - 0.5 * (double)J);
+    kd = fmin(fmax(kd, -1.0e9), 1.0e9);  // NaN -> -1e9, keeps the cast defined
+    return 1 + (long long)kd;
+}
+
+__device__ __forceinline__ int pos_mod(long long k, int K) {
+    long long r = k % (long long)K;
+    return (int)(r < 0 ? r + K : r);
+}
+
+// Coefficients of the J taps along one axis, real kernel.  Returns the grid
+// index of the first tap modulo K.
+template <typename T, int JMAX>
+__device__ __forceinline__ int axis_coefs(T t, int L, const AxisGeom& g,
+                                          const T* __restrict__ hc,
+                                          T (&c)[JMAX]) {
+    const long long k = first_tap(t, g.J);
+    const T tL = (T)L;
+#pragma unroll
+    for (int j = 0; j < JMAX; ++j) {
+        if (j < g.J) {
+            const T p = (t - (T)(k + j)) * tL;
+            const T pc = fmin_t(fmax_t(p, (T)g.nlo), (T)g.nhi);
+            const int n = (int)floor_t(pc);
+            const T alf = p - (T)n;
+            c[j] = ((T)1 - alf) * hc[n] + alf * hc[n + 1];
+        } else {
+            c[j] = (T)0;
+        }
+    }
+    return pos_mod(k, g.K);
+}
+
+// Complex-kernel variant (phasing='complex'): coefficient real and imaginary
+// parts are interpolated separately (nufft_table.template.c:670-672).
+template <typename T, int JMAX>
+__device__ __forceinline__ int axis_coefs(T t, int L, const AxisGeom& g,
+                                          const cx<T>* __restrict__ hc,
+                                          T (&cr)[JMAX], T (&ci)[JMAX]) {
+    const long long k = first_tap(t, g.J);
+    const T tL = (T)L;
+#pragma unroll
+    for (int j = 0; j < JMAX; ++j) {
+        if (j < g.J) {
+            const T p = (t - (T)(k + j)) * tL;
+            const T pc = fmin_t(fmax_t(p, (T)g.nlo), (T)g.nhi);
+            const int n = (int)floor_t(pc);
+            const T alf = p - (T)n;
+            const cx<T> h0 = hc[n];
+            const cx<T> h1 = hc[n + 1];
+            cr[j] = ((T)1 - alf) * h0.x + alf * h1.x;
+            ci[j] = ((T)1 - alf) * h0.y + alf * h1.y;
+        } else {
+            cr[j] = (T)0;
+            ci[j] = (T)0;
+        }
+    }
+    return pos_mod(k, g.K);
+}
+
+// Grid indices of the J taps along one axis (periodic wrap).
+template <int JMAX>
+__device__ __forceinline__ void axis_index(int k0, int J, int K, int (&idx)[JMAX]) {
+    int k = k0;
+#pragma unroll
+    for (int j = 0; j < JMAX; ++j) {
+        idx[j] = k;
+        if (j < J) {
+            ++k;
+            if (k == K) k = 0;
+        }
+    }
+}
+
+// (cos phi, sin phi) of a real phase, evaluated in double and rounded to T.
+template <typename T>
+__device__ __forceinline__ cx<T> cis(T phi) {
+    double s, c;
+    sincos((double)phi, &s, &c);
+    return mkcx<T>((T)c, (T)s);
+}
+
+}  // namespace mi355
