@@ -101,8 +101,44 @@ int mi355nufft_set_points(mi355nufft_plan *plan, const void *omega,
                           int64_t stride_d, const double *shift, int phase_mode,
                           int dot_single);
 
+/*
+ * Points of several frames at once (no counterpart in the reference, which
+ * loops over repetitions in Python, _nufft.py:1069-1084): frame f owns points
+ * [frame_offsets[f], frame_offsets[f+1]) of the concatenated omega (M in
+ * total, uneven per frame), and nframes*nreps grids / images in the transform
+ * calls (index f*nreps + r), while samples stay (nreps, M).  One launch of
+ * each tiled kernel then covers every frame and repetition.
+ *   frame_offsets  host array of nframes+1 int64, 0 = f_0 <= ... <= f_n = M
+ * Other arguments as mi355nufft_set_points (which is the nframes == 1 case).
+ */
+int mi355nufft_set_points_frames(mi355nufft_plan *plan, const void *omega,
+                                 int omega_double, int64_t M, int64_t stride_m,
+                                 int64_t stride_d, const double *shift,
+                                 int phase_mode, int dot_single,
+                                 const int64_t *frame_offsets, int32_t nframes);
+
 /* Number of points set by the last mi355nufft_set_points (0 if none). */
 int64_t mi355nufft_num_points(const mi355nufft_plan *plan);
+
+/*
+ * Bin geometry and kernel choice for the interpolation stages (no counterpart
+ * in the reference, whose kernels run one thread per sample unsorted).
+ *   bin_size    ndim bin sizes in grid points (NULL or 0 entries: defaults
+ *               2-D 32x32, 3-D 16x16x8 / 16x8x8 for complex128, 1-D 512)
+ *   max_points  points per subproblem (0: default 256 in 2-D, 512 else)
+ *   method      0 auto (tiled when the tile fits in LDS), 1 global-memory
+ *               kernels on unsorted points, 2 tiled (error if it cannot)
+ * Re-sorts the current points if any.
+ */
+int mi355nufft_set_binning(mi355nufft_plan *plan, const int32_t *bin_size,
+                           int32_t max_points, int method);
+
+/*
+ * Plan description: info[0..12] = ndim, M, bin size (3), points per
+ * subproblem, number of bins, tile points, tiled kernels active (0/1),
+ * subproblem slots, method, LDS budget in bytes, number of frames.
+ */
+int mi355nufft_plan_info(const mi355nufft_plan *plan, int64_t *info, int32_t n);
 
 /*
  * Forward transform, image -> non-uniform samples.  Replaces nufft_forward
@@ -151,6 +187,20 @@ int mi355nufft_interp(mi355nufft_plan *plan, const void *grid, void *y,
  * (:1101-1191). */
 int mi355nufft_spread(mi355nufft_plan *plan, const void *y, void *grid,
                       int32_t nreps);
+
+/*
+ * Trajectory gradient of the forward interpolation (no counterpart in the
+ * reference, which has no autograd).  With y = mi355nufft_interp(grid):
+ *   grad[m*ndim + d] = sum_r Re(conj(w[r*M + m]) * d y[r*M + m] / d omega[m, d])
+ * including the derivative of phase_after / phase_shift; the table
+ * interpolant is differentiated exactly (slope L*(h[n+1]-h[n]) per cell).
+ *   grid   the forward FFT output (as passed to mi355nufft_interp)
+ *   w      device complex (nreps, M) upstream gradient
+ *   grad   device real (M, ndim), written (float32 / float64 per precision)
+ * Needs the tiled kernels (binning method 0 or 2).
+ */
+int mi355nufft_interp_grad(mi355nufft_plan *plan, const void *grid,
+                           const void *w, void *grad, int32_t nreps);
 
 /* Crop [:N] of the backward-FFT grid, scale, * conj(sn): _nufft.py:1561-1572. */
 int mi355nufft_crop_deapod(mi355nufft_plan *plan, const void *grid, void *x,

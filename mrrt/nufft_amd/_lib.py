@@ -41,13 +41,18 @@ _SIGNATURES = {
     "mi355nufft_plan_destroy": (c_int, [c_vp]),
     "mi355nufft_set_stream": (c_int, [c_vp, c_vp]),
     "mi355nufft_set_points": (c_int, [c_vp, c_vp, c_int, c_i64, c_i64, c_i64, c_dblp, c_int, c_int]),
+    "mi355nufft_set_points_frames": (c_int, [c_vp, c_vp, c_int, c_i64, c_i64, c_i64, c_dblp, c_int, c_int, c_i64p,
+                                             c_i32]),
     "mi355nufft_num_points": (c_i64, [c_vp]),
+    "mi355nufft_set_binning": (c_int, [c_vp, c_i32p, c_i32, c_int]),
+    "mi355nufft_plan_info": (c_int, [c_vp, c_i64p, c_i32]),
     "mi355nufft_forward": (c_int, [c_vp, c_vp, c_i64p, c_vp, c_i32, c_vp]),
     "mi355nufft_adjoint": (c_int, [c_vp, c_vp, c_vp, c_i64p, c_i32, c_vp]),
     "mi355nufft_deapod_pad": (c_int, [c_vp, c_vp, c_i64p, c_vp, c_i32]),
     "mi355nufft_fft": (c_int, [c_vp, c_vp, c_i32, c_int]),
     "mi355nufft_interp": (c_int, [c_vp, c_vp, c_vp, c_i32]),
     "mi355nufft_spread": (c_int, [c_vp, c_vp, c_vp, c_i32]),
+    "mi355nufft_interp_grad": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i32]),
     "mi355nufft_crop_deapod": (c_int, [c_vp, c_vp, c_vp, c_i64p, c_i32]),
     "mi355nufft_table1_forward": (c_int, [c_int, c_i64p, c_i32p, c_i32, c_int, c_int, c_vp, c_vpp, c_vp, c_i64,
                                           c_vp, c_i32, c_vp]),

@@ -249,11 +249,30 @@ class NufftBase(object):
             raise ValueError("omega must be one of the following types: {}".format([np.float32, np.float64]))
         return omega_np, omega_t
 
-    def _set_points(self, omega_np, omega_t):
+    @property
+    def _omega_host(self):
+        if self._omega_np is None:
+            self._omega_np = self._omega_user.cpu().numpy()
+        return self._omega_np
+
+    def set_points(self, omega, frame_offsets=None):
+        """Replace the sampling points (numpy array or device tensor (M, ndim)).
+
+        Runs on the device (tm, per-sample phase, bin sort); the plan's
+        tables, deapodization and FFT plans are kept.  ``frame_offsets``
+        (length nframes + 1) splits the points into frames, each with its own
+        grids (see ``NufftFrames``).
+        """
+        omega_np, omega_t = self._check_omega(omega)
+        self._set_points(omega_np, omega_t, frame_offsets)
+
+    def _set_points(self, omega_np, omega_t, frame_offsets=None):
         self.M = int(omega_t.shape[0])
+        self.nargout1 = self.M
         self._omega_user = omega_t
-        self._omega_host = omega_np if omega_np is not None else omega_t.cpu().numpy()
+        self._omega_np = omega_np
         self._omega_rdt = omega_t.to(self._trdt)
+        self._points_version = getattr(self, "_points_version", 0) + 1
         if self._phasing == "real":
             shift = [float(np.asarray(s - m, dtype=self._real_dtype)) for s, m in zip(self._n_shift, self.n_mid)]
             phase_mode = 1
@@ -268,9 +287,40 @@ class NufftBase(object):
             dot_single = 0
         om = omega_t
         self._sync_stream()
-        _lib.check(self._lib.mi355nufft_set_points(
+        if frame_offsets is None:
+            offs, nframes = None, 1
+        else:
+            offs = _lib.i64arr(list(frame_offsets))
+            nframes = len(frame_offsets) - 1
+        _lib.check(self._lib.mi355nufft_set_points_frames(
             self._plan, ctypes.c_void_p(om.data_ptr()), 1 if om.dtype == torch.float64 else 0, self.M,
-            om.stride(0), om.stride(1), _lib.dblarr(shift + [0.0] * (3 - self.ndim)), phase_mode, dot_single))
+            om.stride(0), om.stride(1), _lib.dblarr(shift + [0.0] * (3 - self.ndim)), phase_mode, dot_single,
+            offs, nframes))
+        self.nframes = nframes
+
+    _METHODS = {"auto": 0, "gm": 1, "tiled": 2}
+
+    def set_binning(self, bin_size=None, max_points=0, method="auto"):
+        """Bin geometry and kernel family of the interpolation stages.
+
+        ``bin_size``: per-axis bin size in grid points (None: defaults);
+        ``max_points``: points per subproblem (0: default); ``method``:
+        'auto' (tiled when the tile fits in LDS), 'gm' (global-memory kernels)
+        or 'tiled'.  The current points are re-sorted.
+        """
+        bs = None
+        if bin_size is not None:
+            bs = _lib.i32arr(_as_tuple(bin_size, type=int, n=self.ndim))
+        self._sync_stream()
+        _lib.check(self._lib.mi355nufft_set_binning(self._plan, bs, int(max_points), self._METHODS[method]))
+
+    def plan_info(self):
+        """dict with the binning state of the plan (see mi355nufft_plan_info)."""
+        buf = (ctypes.c_int64 * 13)()
+        _lib.check(self._lib.mi355nufft_plan_info(self._plan, buf, 13))
+        v = list(buf)
+        return dict(ndim=v[0], M=v[1], bin_size=tuple(v[2:5]), max_points=v[5], nbins=v[6], tile=v[7],
+                    tiled=bool(v[8]), subproblem_slots=v[9], method=v[10], lds_limit=v[11], nframes=v[12])
 
     def _sync_stream(self):
         s = torch.cuda.current_stream(self.device).cuda_stream
@@ -327,7 +377,7 @@ class NufftBase(object):
     def _forward_dev(self, x, nreps):
         """x: device tensor, logical shape Nd + (nreps,) (any strides); returns (nreps, M)."""
         y = torch.empty((nreps, self.M), dtype=self._tcdt, device=self.device)
-        grid = torch.empty((nreps, int(np.prod(self._Kd))), dtype=self._tcdt, device=self.device)
+        grid = torch.empty((self.nframes * nreps, int(np.prod(self._Kd))), dtype=self._tcdt, device=self.device)
         xs = list(x.stride())
         self._sync_stream()
         _lib.check(self._lib.mi355nufft_forward(
@@ -340,7 +390,7 @@ class NufftBase(object):
         if out is None:
             out = torch.empty((nreps,) + tuple(reversed(self._Nd)), dtype=self._tcdt, device=self.device)
             out = out.permute(*(tuple(range(self.ndim, 0, -1)) + (0,)))
-        grid = torch.empty((nreps, int(np.prod(self._Kd))), dtype=self._tcdt, device=self.device)
+        grid = torch.empty((self.nframes * nreps, int(np.prod(self._Kd))), dtype=self._tcdt, device=self.device)
         self._sync_stream()
         _lib.check(self._lib.mi355nufft_adjoint(
             self._plan, ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(out.data_ptr()),

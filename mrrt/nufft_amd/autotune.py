@@ -1,0 +1,127 @@
+"""Autotuner for the tiled interpolation kernels.
+
+Picks the bin size and the points per subproblem (workgroup) of a plan by
+timing the gather and the scatter stages on the plan's own points with HIP
+events, and caches the choice per geometry in a JSON file
+(``$MI355NUFFT_TUNE_CACHE`` or ``~/.cache/mi355nufft/autotune.json``).
+The cache key holds dimensions, grid, kernel width, table size, precision,
+phasing, number of points (rounded to a power of two), repetitions and
+frames; trajectories of the same kind and size share an entry.
+"""
+
+import ctypes
+import json
+import math
+import os
+
+import numpy as np
+import torch
+
+from . import _lib
+
+__all__ = ["tune", "candidates", "cache_path"]
+
+_C2 = [((16, 16), 128), ((16, 16), 256), ((32, 32), 128), ((32, 32), 256), ((32, 32), 512),
+       ((32, 32), 1024), ((64, 32), 256), ((64, 64), 512), ((64, 64), 1024)]
+_C3 = [((8, 8, 8), 256), ((8, 8, 8), 512), ((16, 8, 8), 256), ((16, 8, 8), 512), ((16, 16, 8), 512),
+       ((16, 16, 8), 1024), ((16, 16, 4), 512), ((32, 8, 8), 512)]
+_C1 = [((256,), 256), ((512,), 512), ((1024,), 512), ((2048,), 1024)]
+
+
+def candidates(ndim):
+    return {1: _C1, 2: _C2, 3: _C3}[ndim]
+
+
+def cache_path():
+    p = os.environ.get("MI355NUFFT_TUNE_CACHE")
+    if p:
+        return p
+    return os.path.join(os.path.expanduser("~"), ".cache", "mi355nufft", "autotune.json")
+
+
+def _key(op, nreps):
+    mb = 1 << int(round(math.log2(max(op.M, 1))))
+    return "|".join(str(v) for v in (op.ndim, op.Kd, op.Jd, op.Ld, op.precision, op.phasing, mb, nreps,
+                                     getattr(op, "nframes", 1)))
+
+
+def _load():
+    try:
+        with open(cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(table):
+    p = cache_path()
+    try:
+        os.makedirs(os.path.dirname(p), exist_ok=True)
+        tmp = p + ".tmp%d" % os.getpid()
+        with open(tmp, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+        os.replace(tmp, p)
+    except OSError:
+        pass
+
+
+def _time(fn, reps):
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    fn()
+    best = float("inf")
+    for _ in range(reps):
+        s.record()
+        fn()
+        e.record()
+        e.synchronize()
+        best = min(best, s.elapsed_time(e))
+    return best
+
+
+def tune(op, nreps=1, cands=None, reps=3, use_cache=True, verbose=False):
+    """Choose bin size / points per subproblem for ``op`` (a NufftBase, or
+    the ``.op`` of a NufftFrames) with ``nreps`` repetitions per frame.
+
+    Returns ``(bin_size, max_points, timings)``; the plan is left configured
+    with the choice.  ``timings`` maps each candidate to (gather_ms, scatter_ms).
+    """
+    key = _key(op, nreps)
+    table = _load() if use_cache else {}
+    if use_cache and key in table:
+        b, p = table[key]["bin_size"], table[key]["max_points"]
+        op.set_binning(tuple(b), p, "auto")
+        return tuple(b), p, table[key].get("timings", {})
+    cands = cands or candidates(op.ndim)
+    nimg = getattr(op, "nframes", 1) * nreps
+    nk = int(np.prod(op.Kd))
+    g = torch.randn((nimg, nk), dtype=op._tcdt, device=op.device)
+    y = torch.randn((nreps, op.M), dtype=op._tcdt, device=op.device)
+    lib = op._lib
+    vp = ctypes.c_void_p
+    timings = {}
+    best = None
+    for b, p in cands:
+        try:
+            op.set_binning(b, p, "tiled")
+        except ValueError:
+            continue
+        op._sync_stream()
+        tf = _time(lambda: _lib.check(lib.mi355nufft_interp(op._plan, vp(g.data_ptr()), vp(y.data_ptr()), nreps)),
+                   reps)
+        ts = _time(lambda: _lib.check(lib.mi355nufft_spread(op._plan, vp(y.data_ptr()), vp(g.data_ptr()), nreps)),
+                   reps)
+        timings[str((b, p))] = (tf, ts)
+        if verbose:
+            print("autotune", b, p, "gather %.4f ms scatter %.4f ms" % (tf, ts))
+        if best is None or tf + ts < best[0]:
+            best = (tf + ts, b, p)
+    if best is None:
+        op.set_binning(None, 0, "auto")
+        return None, 0, timings
+    _, b, p = best
+    op.set_binning(b, p, "auto")
+    if use_cache:
+        table[key] = {"bin_size": list(b), "max_points": p, "timings": timings}
+        _store(table)
+    return tuple(b), p, timings

@@ -118,6 +118,58 @@ __device__ __forceinline__ int axis_coefs(T t, int L, const AxisGeom& g,
     return pos_mod(k, g.K);
 }
 
+// Coefficients and their t-derivatives.  The derivative is that of the
+// piecewise-linear table interpolant itself, L * (h[n+1] - h[n]) on the
+// cell used by the coefficient, so it is exact for the operator as
+// evaluated (finite differences agree away from cell boundaries).
+template <typename T, int JMAX>
+__device__ __forceinline__ int axis_coefs_d(T t, int L, const AxisGeom& g, const T* __restrict__ hc,
+                                            T (&c)[JMAX], T (&dc)[JMAX]) {
+    const long long k = first_tap(t, g.J);
+    const T tL = (T)L;
+#pragma unroll
+    for (int j = 0; j < JMAX; ++j) {
+        if (j < g.J) {
+            const T p = (t - (T)(k + j)) * tL;
+            const T pc = fmin_t(fmax_t(p, (T)g.nlo), (T)g.nhi);
+            const int n = (int)floor_t(pc);
+            const T alf = p - (T)n;
+            const T h0 = hc[n], h1 = hc[n + 1];
+            c[j] = ((T)1 - alf) * h0 + alf * h1;
+            dc[j] = tL * (h1 - h0);
+        } else {
+            c[j] = (T)0;
+            dc[j] = (T)0;
+        }
+    }
+    return pos_mod(k, g.K);
+}
+
+template <typename T, int JMAX>
+__device__ __forceinline__ int axis_coefs_d(T t, int L, const AxisGeom& g, const cx<T>* __restrict__ hc,
+                                            T (&cr)[JMAX], T (&ci)[JMAX], T (&dcr)[JMAX], T (&dci)[JMAX]) {
+    const long long k = first_tap(t, g.J);
+    const T tL = (T)L;
+#pragma unroll
+    for (int j = 0; j < JMAX; ++j) {
+        if (j < g.J) {
+            const T p = (t - (T)(k + j)) * tL;
+            const T pc = fmin_t(fmax_t(p, (T)g.nlo), (T)g.nhi);
+            const int n = (int)floor_t(pc);
+            const T alf = p - (T)n;
+            const cx<T> h0 = hc[n];
+            const cx<T> h1 = hc[n + 1];
+            cr[j] = ((T)1 - alf) * h0.x + alf * h1.x;
+            ci[j] = ((T)1 - alf) * h0.y + alf * h1.y;
+            dcr[j] = tL * (h1.x - h0.x);
+            dci[j] = tL * (h1.y - h0.y);
+        } else {
+            cr[j] = ci[j] = dcr[j] = dci[j] = (T)0;
+        }
+    }
+    return pos_mod(k, g.K);
+}
+
 // Grid indices of the J taps along one axis (periodic wrap).
 template <int JMAX>
 __device__ __forceinline__ void axis_index(int k0, int J, int K, int (&idx)[JMAX]) {

@@ -1,0 +1,524 @@
+// Bin-sorted, LDS-tiled table1 kernels.
+//
+// Points are sorted by the bin of their first tap (the low corner of their
+// J^d footprint) on a grid of B_0 x B_1 x B_2 bins.  A bin's footprints lie in
+// a tile of (B_d + J_d - 1) points per axis that starts at the bin's corner,
+// so all taps of a bin address the tile with local, wrap-free indices; the
+// periodic wrap happens once per tile point when it is staged or flushed.
+// Each bin is cut into subproblems of at most P points; one workgroup runs
+// one subproblem:
+//   gather:  stage the tile (grid * phase_before / scale_ortho) in LDS, then
+//            every point sums its taps from LDS (the reference's sum order);
+//   scatter: accumulate every point's taps into an LDS tile with LDS atomics,
+//            then flush the non-zero tile points to the grid with one global
+//            atomic per component, multiplied by conj(phase_before).
+// The launch grid is an upper bound on the subproblem count (bins + M/P),
+// so no host synchronisation is needed; surplus workgroups exit at once.
+#pragma once
+
+#include "device_common.hpp"
+#include "kernels_grid.hpp"
+#include "kernels_interp.hpp"
+
+namespace mi355 {
+
+struct BinGeom {
+    int B[3];       // bin size per axis
+    int nb[3];      // bins per axis, ceil(K / B)
+    int T[3];       // tile size per axis, B + J - 1
+    int tile;       // T0 * T1 * T2
+    int nbins;      // bins of all frames: nframes * nbins_frame
+    int nbins_frame;  // nb0 * nb1 * nb2
+    int P;          // max points per subproblem
+};
+
+struct BinState {
+    int* bin_of;     // (M) bin of each point
+    int* rank;       // (M) rank inside its bin
+    int* counts;     // (nbins) points per bin
+    int* bin_start;  // (nbins) first sorted position of each bin
+    int* perm;       // (M) sorted position -> point index
+    int* sub_bin;    // (max_sub) bin of each subproblem
+    int* sub_start;  // (max_sub) first sorted position of each subproblem
+    int* n_sub;      // (1) number of subproblems
+    const int64_t* frame_off;  // (nframes + 1) first point of each frame, or null
+    int nframes;
+};
+
+template <typename T>
+__device__ __forceinline__ int first_tap_mod(T t, int J, int K) {
+    return pos_mod(first_tap(t, J), K);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void bin_count_kernel(const T* __restrict__ tm, int64_t M, int ndim,
+                                                        AxisGeom a0, AxisGeom a1, AxisGeom a2, BinGeom bg,
+                                                        BinState bs) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    const AxisGeom ax[3] = {a0, a1, a2};
+    int bin = 0, mul = 1;
+    for (int d = 0; d < ndim; ++d) {
+        const int k0 = first_tap_mod(tm[(int64_t)d * M + m], ax[d].J, ax[d].K);
+        bin += (k0 / bg.B[d]) * mul;
+        mul *= bg.nb[d];
+    }
+    if (bs.frame_off) {  // frame of point m: last f with frame_off[f] <= m
+        int lo = 0, hi = bs.nframes - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (bs.frame_off[mid] <= m) lo = mid;
+            else hi = mid - 1;
+        }
+        bin += lo * bg.nbins_frame;
+    }
+    bs.bin_of[m] = bin;
+    bs.rank[m] = atomicAdd(&bs.counts[bin], 1);
+}
+
+// One workgroup of 1024 threads: exclusive scans of the bin counts and of the
+// per-bin subproblem counts, then the subproblem table.
+__global__ __launch_bounds__(1024) void bin_scan_kernel(BinState bs, int nbins, int P, int max_sub) {
+    __shared__ int s_pts[1024];
+    __shared__ int s_sub[1024];
+    const int tid = threadIdx.x;
+    const int chunk = (nbins + 1023) / 1024;
+    const int b0 = tid * chunk;
+    const int b1 = min(nbins, b0 + chunk);
+    int pts = 0, subs = 0;
+    for (int b = b0; b < b1; ++b) {
+        const int c = bs.counts[b];
+        pts += c;
+        subs += (c + P - 1) / P;
+    }
+    s_pts[tid] = pts;
+    s_sub[tid] = subs;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
+        int vp = 0, vs = 0;
+        if (tid >= off) {
+            vp = s_pts[tid - off];
+            vs = s_sub[tid - off];
+        }
+        __syncthreads();
+        s_pts[tid] += vp;
+        s_sub[tid] += vs;
+        __syncthreads();
+    }
+    int run_p = s_pts[tid] - pts;
+    int run_s = s_sub[tid] - subs;
+    for (int b = b0; b < b1; ++b) {
+        const int c = bs.counts[b];
+        bs.bin_start[b] = run_p;
+        const int ns = (c + P - 1) / P;
+        for (int i = 0; i < ns && run_s + i < max_sub; ++i) {
+            bs.sub_bin[run_s + i] = b;
+            bs.sub_start[run_s + i] = run_p + i * P;
+        }
+        run_p += c;
+        run_s += ns;
+    }
+    if (tid == 1023) bs.n_sub[0] = min(s_sub[1023], max_sub);
+}
+
+__global__ __launch_bounds__(256) void bin_scatter_kernel(int64_t M, BinState bs) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    bs.perm[bs.bin_start[bs.bin_of[m]] + bs.rank[m]] = (int)m;
+}
+
+__global__ __launch_bounds__(256) void fill_int_kernel(int* __restrict__ p, int64_t n, int v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <typename T, bool CK>
+struct TiledArgs {
+    InterpArgs<T, CK> ia;   // tables, tm, samples, grid, per-sample phase, geometry
+    BinGeom bg;
+    BinState bs;
+    T pb[3];                // phase_before coefficients (real kernel)
+    int use_pb;             // multiply by phase_before (gather) / conj (scatter)
+    int ortho;              // gather: divide staged values by scale_ortho
+    T scale_ortho;
+};
+
+struct Sub {
+    int start, end;
+    int frame;  // frame of the bin (0 without frames)
+    int o[3];   // tile origin (grid coordinates)
+};
+
+// Subproblem sid; the workgroups of a launch stride over the subproblem list
+// (persistent launch), so every workgroup exits after n_sub is exhausted.
+__device__ __forceinline__ void load_sub(const BinGeom& bg, const BinState& bs, int ndim, int sid, Sub& s) {
+    const int bin = bs.sub_bin[sid];
+    s.start = bs.sub_start[sid];
+    s.end = min(s.start + bg.P, bs.bin_start[bin] + bs.counts[bin]);
+    s.frame = bin / bg.nbins_frame;
+    int q = bin - s.frame * bg.nbins_frame;
+    for (int d = 0; d < 3; ++d) {
+        const int bd = d < ndim ? q % bg.nb[d] : 0;
+        if (d < ndim) q /= bg.nb[d];
+        s.o[d] = bd * bg.B[d];
+    }
+}
+
+// grid index of local tile point e
+__device__ __forceinline__ int64_t tile_to_grid(int e, const BinGeom& bg, const Sub& s, const AxisGeom* ax,
+                                                int& g0, int& g1, int& g2) {
+    const int e0 = e % bg.T[0];
+    const int q = e / bg.T[0];
+    const int e1 = q % bg.T[1];
+    const int e2 = q / bg.T[1];
+    g0 = (s.o[0] + e0) % ax[0].K;
+    g1 = (s.o[1] + e1) % ax[1].K;
+    g2 = (s.o[2] + e2) % ax[2].K;
+    return ((int64_t)g2 * ax[1].K + g1) * ax[0].K + g0;
+}
+
+// Stage the tile of subproblem s from grid g into LDS: grid * phase_before
+// (real kernel) / scale_ortho (ortho), the order of _nufft.py:1370-1375.
+template <typename T, int NDIM, bool CK>
+__device__ __forceinline__ void stage_tile(const TiledArgs<T, CK>& a, const Sub& s, const AxisGeom* ax,
+                                           const cx<T>* __restrict__ g, cx<T>* tile) {
+    const BinGeom& bg = a.bg;
+    for (int e = threadIdx.x; e < bg.tile; e += blockDim.x) {
+        int g0, g1, g2;
+        const int64_t gi = tile_to_grid(e, bg, s, ax, g0, g1, g2);
+        cx<T> v = g[gi];
+        if (a.use_pb) {
+            const T c[3] = {a.pb[0], a.pb[1], a.pb[2]};
+            const cx<T> p = phase_before_at<T>(c, NDIM, g0, g1, g2);
+            const T vr = v.x * p.x - v.y * p.y;
+            const T vi = v.x * p.y + v.y * p.x;
+            v.x = vr;
+            v.y = vi;
+        }
+        if (a.ortho) {
+            v.x = v.x / a.scale_ortho;
+            v.y = v.y / a.scale_ortho;
+        }
+        tile[e] = v;
+    }
+}
+
+template <typename T, int NDIM, bool CK, int JMAX>
+__global__ __launch_bounds__(256) void interp_tiled_kernel(TiledArgs<T, CK> a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* tile = reinterpret_cast<cx<T>*>(smem);
+    const InterpArgs<T, CK>& ia = a.ia;
+    const BinGeom& bg = a.bg;
+    const AxisGeom ax[3] = {ia.ax[0], ia.ax[1], ia.ax[2]};
+    const int T0 = bg.T[0], T1 = bg.T[1];
+    const int tid = threadIdx.x;
+    const int n_sub = a.bs.n_sub[0];
+    for (int sid = blockIdx.x; sid < n_sub; sid += gridDim.x) {
+    Sub s;
+    load_sub(bg, a.bs, NDIM, sid, s);
+    for (int r = 0; r < ia.nreps; ++r) {
+        stage_tile<T, NDIM, CK>(a, s, ax, ia.grid + ((int64_t)s.frame * ia.nreps + r) * ia.kstride, tile);
+        __syncthreads();
+        for (int i = s.start + tid; i < s.end; i += blockDim.x) {
+            const int64_t m = a.bs.perm[i];
+            T cr[3][JMAX], ci[3][JMAX];
+            int l[3];
+#pragma unroll
+            for (int d = 0; d < NDIM; ++d) {
+                const T t = ia.tm[(int64_t)d * ia.M + m];
+                int k0;
+                if constexpr (CK) k0 = axis_coefs<T, JMAX>(t, ia.L, ax[d], ia.h[d], cr[d], ci[d]);
+                else k0 = axis_coefs<T, JMAX>(t, ia.L, ax[d], ia.h[d], cr[d]);
+                l[d] = k0 - s.o[d];
+            }
+            T sr = 0, si = 0;
+            if constexpr (NDIM == 1) {
+#pragma unroll
+                for (int j1 = 0; j1 < JMAX; ++j1)
+                    if (j1 < ax[0].J) {
+                        const cx<T> v = tile[l[0] + j1];
+                        madd<T, CK>(sr, si, cr[0][j1], ci[0][j1], v.x, v.y);
+                    }
+            } else if constexpr (NDIM == 2) {
+#pragma unroll
+                for (int j2 = 0; j2 < JMAX; ++j2)
+                    if (j2 < ax[1].J) {
+                        const cx<T>* row = tile + (l[1] + j2) * T0 + l[0];
+                        T s1r = 0, s1i = 0;
+#pragma unroll
+                        for (int j1 = 0; j1 < JMAX; ++j1)
+                            if (j1 < ax[0].J) {
+                                const cx<T> v = row[j1];
+                                madd<T, CK>(s1r, s1i, cr[0][j1], ci[0][j1], v.x, v.y);
+                            }
+                        madd<T, CK>(sr, si, cr[1][j2], ci[1][j2], s1r, s1i);
+                    }
+            } else {
+                for (int j3 = 0; j3 < JMAX; ++j3)
+                    if (j3 < ax[2].J) {
+                        T s2r = 0, s2i = 0;
+#pragma unroll
+                        for (int j2 = 0; j2 < JMAX; ++j2)
+                            if (j2 < ax[1].J) {
+                                const cx<T>* row = tile + ((l[2] + j3) * T1 + (l[1] + j2)) * T0 + l[0];
+                                T s1r = 0, s1i = 0;
+#pragma unroll
+                                for (int j1 = 0; j1 < JMAX; ++j1)
+                                    if (j1 < ax[0].J) {
+                                        const cx<T> v = row[j1];
+                                        madd<T, CK>(s1r, s1i, cr[0][j1], ci[0][j1], v.x, v.y);
+                                    }
+                                madd<T, CK>(s2r, s2i, cr[1][j2], ci[1][j2], s1r, s1i);
+                            }
+                        madd<T, CK>(sr, si, cr[2][j3], ci[2][j3], s2r, s2i);
+                    }
+            }
+            cx<T> o;
+            if (ia.phase) {
+                const cx<T> ph = ia.phase[m];
+                o.x = sr * ph.x - si * ph.y;
+                o.y = sr * ph.y + si * ph.x;
+            } else {
+                o.x = sr;
+                o.y = si;
+            }
+            ia.out[(int64_t)r * ia.M + m] = o;
+        }
+        __syncthreads();
+    }
+    }
+}
+
+template <typename T, int NDIM, bool CK, int JMAX>
+__global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const InterpArgs<T, CK>& ia = a.ia;
+    const BinGeom& bg = a.bg;
+    T* tre = reinterpret_cast<T*>(smem);
+    T* tim = tre + bg.tile;
+    const AxisGeom ax[3] = {ia.ax[0], ia.ax[1], ia.ax[2]};
+    const int T0 = bg.T[0], T1 = bg.T[1];
+    const int tid = threadIdx.x;
+    const int n_sub = a.bs.n_sub[0];
+    for (int sid = blockIdx.x; sid < n_sub; sid += gridDim.x) {
+    Sub s;
+    load_sub(bg, a.bs, NDIM, sid, s);
+    for (int r = 0; r < ia.nreps; ++r) {
+        for (int e = tid; e < bg.tile; e += blockDim.x) {
+            tre[e] = (T)0;
+            tim[e] = (T)0;
+        }
+        __syncthreads();
+        for (int i = s.start + tid; i < s.end; i += blockDim.x) {
+            const int64_t m = a.bs.perm[i];
+            T cr[3][JMAX], ci[3][JMAX];
+            int l[3];
+#pragma unroll
+            for (int d = 0; d < NDIM; ++d) {
+                const T t = ia.tm[(int64_t)d * ia.M + m];
+                int k0;
+                if constexpr (CK) k0 = axis_coefs<T, JMAX>(t, ia.L, ax[d], ia.h[d], cr[d], ci[d]);
+                else k0 = axis_coefs<T, JMAX>(t, ia.L, ax[d], ia.h[d], cr[d]);
+                l[d] = k0 - s.o[d];
+            }
+            cx<T> f = ia.in[(int64_t)r * ia.M + m];
+            if (ia.phase) {
+                const cx<T> ph = ia.phase[m];
+                const T fr = f.x * ph.x + f.y * ph.y;
+                const T fi = f.y * ph.x - f.x * ph.y;
+                f.x = fr;
+                f.y = fi;
+            }
+            if constexpr (NDIM == 1) {
+#pragma unroll
+                for (int j1 = 0; j1 < JMAX; ++j1)
+                    if (j1 < ax[0].J) {
+                        T vr, vi;
+                        cmul_conj<T, CK>(vr, vi, cr[0][j1], ci[0][j1], f.x, f.y);
+                        atomicAdd(tre + l[0] + j1, vr);
+                        atomicAdd(tim + l[0] + j1, vi);
+                    }
+            } else if constexpr (NDIM == 2) {
+#pragma unroll
+                for (int j2 = 0; j2 < JMAX; ++j2)
+                    if (j2 < ax[1].J) {
+                        T v2r, v2i;
+                        cmul_conj<T, CK>(v2r, v2i, cr[1][j2], ci[1][j2], f.x, f.y);
+                        const int row = (l[1] + j2) * T0 + l[0];
+#pragma unroll
+                        for (int j1 = 0; j1 < JMAX; ++j1)
+                            if (j1 < ax[0].J) {
+                                T vr, vi;
+                                cmul_conj<T, CK>(vr, vi, cr[0][j1], ci[0][j1], v2r, v2i);
+                                atomicAdd(tre + row + j1, vr);
+                                atomicAdd(tim + row + j1, vi);
+                            }
+                    }
+            } else {
+                for (int j3 = 0; j3 < JMAX; ++j3)
+                    if (j3 < ax[2].J) {
+                        T v3r, v3i;
+                        cmul_conj<T, CK>(v3r, v3i, cr[2][j3], ci[2][j3], f.x, f.y);
+#pragma unroll
+                        for (int j2 = 0; j2 < JMAX; ++j2)
+                            if (j2 < ax[1].J) {
+                                T v2r, v2i;
+                                cmul_conj<T, CK>(v2r, v2i, cr[1][j2], ci[1][j2], v3r, v3i);
+                                const int row = ((l[2] + j3) * T1 + (l[1] + j2)) * T0 + l[0];
+#pragma unroll
+                                for (int j1 = 0; j1 < JMAX; ++j1)
+                                    if (j1 < ax[0].J) {
+                                        T vr, vi;
+                                        cmul_conj<T, CK>(vr, vi, cr[0][j1], ci[0][j1], v2r, v2i);
+                                        atomicAdd(tre + row + j1, vr);
+                                        atomicAdd(tim + row + j1, vi);
+                                    }
+                            }
+                    }
+            }
+        }
+        __syncthreads();
+        cx<T>* __restrict__ g = ia.grid_out + ((int64_t)s.frame * ia.nreps + r) * ia.kstride;
+        for (int e = tid; e < bg.tile; e += blockDim.x) {
+            T vr = tre[e], vi = tim[e];
+            if (vr == (T)0 && vi == (T)0) continue;
+            int g0, g1, g2;
+            const int64_t gi = tile_to_grid(e, bg, s, ax, g0, g1, g2);
+            if (a.use_pb) {  // * conj(phase_before)
+                const T c[3] = {a.pb[0], a.pb[1], a.pb[2]};
+                const cx<T> p = phase_before_at<T>(c, NDIM, g0, g1, g2);
+                const T wr = vr * p.x + vi * p.y;
+                const T wi = vi * p.x - vr * p.y;
+                vr = wr;
+                vi = wi;
+            }
+            T* q = reinterpret_cast<T*>(g + gi);
+            unsafeAtomicAdd(q, vr);
+            unsafeAtomicAdd(q + 1, vi);
+        }
+        __syncthreads();
+    }
+    }
+}
+
+// Trajectory gradient.  For y[r, m] = phase[m] * sum_k psi(t_m - k) G_r[k]
+// (G_r the staged grid), t_d = omega_d * K_d / (2 pi), phase = exp(i omega . s):
+//   grad[m, d] = sum_r Re(conj(w[r, m]) * dy[r, m] / d omega_d)
+//   dy/d omega_d = phase * (K_d / 2 pi) * sum_k d_d psi(t_m - k) G_r[k] + i s_d y.
+// Every point belongs to exactly one subproblem, so its thread owns grad[m, :]
+// and accumulates the repetitions without atomics.
+template <typename T>
+struct GradArgs {
+    const cx<T>* w;   // (nreps, M) upstream gradient
+    T* grad;          // (M, ndim) output, row-major like omega
+    T dtdw[3];        // K_d / (2 pi)
+    T shift[3];       // phase exponent coefficients s_d (0 if no phase)
+};
+
+template <typename T, bool CK>
+__device__ __forceinline__ void cmadd(T& ar, T& ai, T cr, T ci, T vr, T vi) {
+    madd<T, CK>(ar, ai, cr, ci, vr, vi);
+}
+
+template <typename T, int NDIM, bool CK, int JMAX>
+__global__ __launch_bounds__(256) void grad_tiled_kernel(TiledArgs<T, CK> a, GradArgs<T> gr) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* tile = reinterpret_cast<cx<T>*>(smem);
+    const InterpArgs<T, CK>& ia = a.ia;
+    const BinGeom& bg = a.bg;
+    const AxisGeom ax[3] = {ia.ax[0], ia.ax[1], ia.ax[2]};
+    const int T0 = bg.T[0], T1 = bg.T[1];
+    const int tid = threadIdx.x;
+    const int n_sub = a.bs.n_sub[0];
+    for (int sid = blockIdx.x; sid < n_sub; sid += gridDim.x) {
+    Sub s;
+    load_sub(bg, a.bs, NDIM, sid, s);
+    for (int r = 0; r < ia.nreps; ++r) {
+        stage_tile<T, NDIM, CK>(a, s, ax, ia.grid + ((int64_t)s.frame * ia.nreps + r) * ia.kstride, tile);
+        __syncthreads();
+        for (int i = s.start + tid; i < s.end; i += blockDim.x) {
+            const int64_t m = a.bs.perm[i];
+            T cr[3][JMAX], ci[3][JMAX], dr[3][JMAX], di[3][JMAX];
+            int l[3];
+#pragma unroll
+            for (int d = 0; d < NDIM; ++d) {
+                const T t = ia.tm[(int64_t)d * ia.M + m];
+                int k0;
+                if constexpr (CK) k0 = axis_coefs_d<T, JMAX>(t, ia.L, ax[d], ia.h[d], cr[d], ci[d], dr[d], di[d]);
+                else k0 = axis_coefs_d<T, JMAX>(t, ia.L, ax[d], ia.h[d], cr[d], dr[d]);
+                l[d] = k0 - s.o[d];
+            }
+            // S[0] = value, S[1 + d] = derivative along axis d
+            T Sr[4] = {0, 0, 0, 0}, Si[4] = {0, 0, 0, 0};
+            if constexpr (NDIM == 1) {
+#pragma unroll
+                for (int j1 = 0; j1 < JMAX; ++j1)
+                    if (j1 < ax[0].J) {
+                        const cx<T> v = tile[l[0] + j1];
+                        cmadd<T, CK>(Sr[0], Si[0], cr[0][j1], ci[0][j1], v.x, v.y);
+                        cmadd<T, CK>(Sr[1], Si[1], dr[0][j1], di[0][j1], v.x, v.y);
+                    }
+            } else if constexpr (NDIM == 2) {
+#pragma unroll
+                for (int j2 = 0; j2 < JMAX; ++j2)
+                    if (j2 < ax[1].J) {
+                        const cx<T>* row = tile + (l[1] + j2) * T0 + l[0];
+                        T ar = 0, ai = 0, br = 0, bi = 0;
+#pragma unroll
+                        for (int j1 = 0; j1 < JMAX; ++j1)
+                            if (j1 < ax[0].J) {
+                                const cx<T> v = row[j1];
+                                cmadd<T, CK>(ar, ai, cr[0][j1], ci[0][j1], v.x, v.y);
+                                cmadd<T, CK>(br, bi, dr[0][j1], di[0][j1], v.x, v.y);
+                            }
+                        cmadd<T, CK>(Sr[0], Si[0], cr[1][j2], ci[1][j2], ar, ai);
+                        cmadd<T, CK>(Sr[1], Si[1], cr[1][j2], ci[1][j2], br, bi);
+                        cmadd<T, CK>(Sr[2], Si[2], dr[1][j2], di[1][j2], ar, ai);
+                    }
+            } else {
+                for (int j3 = 0; j3 < JMAX; ++j3)
+                    if (j3 < ax[2].J) {
+                        T pr = 0, pi = 0, qr = 0, qi = 0, ur = 0, ui = 0;
+#pragma unroll
+                        for (int j2 = 0; j2 < JMAX; ++j2)
+                            if (j2 < ax[1].J) {
+                                const cx<T>* row = tile + ((l[2] + j3) * T1 + (l[1] + j2)) * T0 + l[0];
+                                T ar = 0, ai = 0, br = 0, bi = 0;
+#pragma unroll
+                                for (int j1 = 0; j1 < JMAX; ++j1)
+                                    if (j1 < ax[0].J) {
+                                        const cx<T> v = row[j1];
+                                        cmadd<T, CK>(ar, ai, cr[0][j1], ci[0][j1], v.x, v.y);
+                                        cmadd<T, CK>(br, bi, dr[0][j1], di[0][j1], v.x, v.y);
+                                    }
+                                cmadd<T, CK>(pr, pi, cr[1][j2], ci[1][j2], ar, ai);
+                                cmadd<T, CK>(qr, qi, cr[1][j2], ci[1][j2], br, bi);
+                                cmadd<T, CK>(ur, ui, dr[1][j2], di[1][j2], ar, ai);
+                            }
+                        cmadd<T, CK>(Sr[0], Si[0], cr[2][j3], ci[2][j3], pr, pi);
+                        cmadd<T, CK>(Sr[1], Si[1], cr[2][j3], ci[2][j3], qr, qi);
+                        cmadd<T, CK>(Sr[2], Si[2], cr[2][j3], ci[2][j3], ur, ui);
+                        cmadd<T, CK>(Sr[3], Si[3], dr[2][j3], di[2][j3], pr, pi);
+                    }
+            }
+            cx<T> ph = mkcx<T>((T)1, (T)0);
+            if (ia.phase) ph = ia.phase[m];
+            // y = S0 * ph
+            const T yr = Sr[0] * ph.x - Si[0] * ph.y;
+            const T yi = Sr[0] * ph.y + Si[0] * ph.x;
+            const cx<T> w = gr.w[(int64_t)r * ia.M + m];
+#pragma unroll
+            for (int d = 0; d < NDIM; ++d) {
+                const T er = (Sr[1 + d] * ph.x - Si[1 + d] * ph.y) * gr.dtdw[d] - gr.shift[d] * yi;
+                const T ei = (Sr[1 + d] * ph.y + Si[1 + d] * ph.x) * gr.dtdw[d] + gr.shift[d] * yr;
+                const T v = w.x * er + w.y * ei;
+                T* o = gr.grad + m * NDIM + d;
+                *o = (r == 0) ? v : *o + v;
+            }
+        }
+        __syncthreads();
+    }
+    }
+}
+
+}  // namespace mi355

@@ -16,6 +16,7 @@
 #include "../../../include/mi355nufft.h"
 #include "kernels_grid.hpp"
 #include "kernels_interp.hpp"
+#include "kernels_binned.hpp"
 
 using namespace mi355;
 
@@ -85,9 +86,25 @@ struct mi355nufft_plan {
     void* d_tm = nullptr;     // (ndim, M) real
     void* d_phase = nullptr;  // (M) complex or null
     int has_phase = 0;
+    double shift[3] = {0, 0, 0};  // exponent coefficients of the per-sample phase
 
     void* d_grid = nullptr;  // plan-owned grid scratch
     size_t grid_bytes = 0;
+
+    // binning (tiled kernels)
+    int method = 0;           // 0 auto, 1 global-memory kernels, 2 tiled kernels
+    int bin_req[3] = {0, 0, 0};  // requested bin size per axis (0: default)
+    int P_req = 0;               // requested points per subproblem (0: default)
+    BinGeom bg{};
+    int binned = 0;           // bin state valid for the current points
+    int* d_bin_i = nullptr;   // one allocation: bin_of, rank, perm (3*M_cap)
+    int* d_bin_b = nullptr;   // counts, bin_start (2*nbins_cap)
+    int* d_sub = nullptr;     // sub_bin, sub_start (2*sub_cap) + n_sub
+    int64_t bin_m_cap = 0, bin_b_cap = 0, bin_s_cap = 0;
+    size_t lds_limit = 65536;
+    int nframes = 1;              // frames with their own points (grids per frame: nreps)
+    int64_t* d_frame_off = nullptr;
+    int64_t frame_cap = 0;
 
     std::vector<FftPlans> ffts;
     rocfft_execution_info fft_info = nullptr;
@@ -373,9 +390,17 @@ int crop_deapod_impl(mi355nufft_plan* p, const void* grid, void* x, const int64_
 }
 
 template <typename T>
+int run_tiled_any(mi355nufft_plan* p, bool adjoint, const void* samples_in, void* samples_out, const void* grid_in,
+                  void* grid_out, int32_t nreps);
+bool use_tiled(const mi355nufft_plan* p);
+
+template <typename T>
 int interp_impl(mi355nufft_plan* p, const void* grid, void* y, int32_t nreps) {
-    // phase_before (and 1/sqrt(prod K) if ortho) is applied in place on the
-    // grid, which is the plan's FFT output, before the gather
+    // tiled path: phase_before and the ortho scale are applied while staging
+    if (use_tiled(p)) return run_tiled_any<T>(p, false, nullptr, y, grid, nullptr, nreps);
+    if (p->nframes > 1) return fail(MI355NUFFT_ESTATE, "frames need the tiled kernels");
+    // global-memory path: phase_before (and 1/sqrt(prod K) if ortho) is
+    // applied in place on the grid before the gather
     if (p->phasing == 0 || p->ortho)
         RET_IF(phase_grid_impl<T>(p, const_cast<void*>(grid), nreps, p->phasing == 0, 0, p->ortho));
     int32_t J[3] = {p->J[0], p->J[1], p->J[2]};
@@ -386,7 +411,9 @@ int interp_impl(mi355nufft_plan* p, const void* grid, void* y, int32_t nreps) {
 
 template <typename T>
 int spread_impl(mi355nufft_plan* p, const void* y, void* grid, int32_t nreps) {
-    RET_IF(zero_grid<T>(grid, (int64_t)nreps * p->nk, p->stream));
+    RET_IF(zero_grid<T>(grid, (int64_t)p->nframes * nreps * p->nk, p->stream));
+    if (use_tiled(p)) return run_tiled_any<T>(p, true, y, nullptr, nullptr, grid, nreps);
+    if (p->nframes > 1) return fail(MI355NUFFT_ESTATE, "frames need the tiled kernels");
     int32_t J[3] = {p->J[0], p->J[1], p->J[2]};
     RET_IF(run_table1<T>(true, p->ndim, p->K, J, p->L, p->phasing, y, nullptr, nullptr, grid,
                          (const void* const*)p->d_tab, p->d_tm, p->M, nreps,
@@ -403,6 +430,224 @@ int fft_impl(mi355nufft_plan* p, void* grid, int32_t nreps, int direction) {
     void* bufs[1] = {grid};
     FFT_TRY(rocfft_execute(direction < 0 ? f->fwd : f->bwd, bufs, nullptr, p->fft_info));
     return MI355NUFFT_OK;
+}
+
+// Default bin geometry: 2-D 32x32 bins, 3-D 16x16x8 (complex64) or 16x8x8
+// (complex128), 1-D 512; P points per subproblem.  Requested sizes override.
+void default_bins(const mi355nufft_plan* p, int B[3], int& P) {
+    for (int d = 0; d < 3; ++d) B[d] = 1;
+    if (p->ndim == 1) {
+        B[0] = 512;
+        P = 512;
+    } else if (p->ndim == 2) {
+        B[0] = 32;
+        B[1] = 32;
+        P = 256;
+    } else {
+        B[0] = 16;
+        B[1] = p->precision == 0 ? 16 : 8;
+        B[2] = 8;
+        P = 512;
+    }
+    for (int d = 0; d < p->ndim; ++d)
+        if (p->bin_req[d] > 0) B[d] = p->bin_req[d];
+    if (p->P_req > 0) P = p->P_req;
+}
+
+void setup_bin_geom(mi355nufft_plan* p) {
+    BinGeom& bg = p->bg;
+    int B[3], P;
+    default_bins(p, B, P);
+    bg.nbins = 1;
+    bg.tile = 1;
+    for (int d = 0; d < 3; ++d) {
+        if (d < p->ndim) {
+            bg.B[d] = B[d] < (int)p->K[d] ? B[d] : (int)p->K[d];
+            bg.nb[d] = (int)((p->K[d] + bg.B[d] - 1) / bg.B[d]);
+            bg.T[d] = bg.B[d] + p->J[d] - 1;
+        } else {
+            bg.B[d] = 1;
+            bg.nb[d] = 1;
+            bg.T[d] = 1;
+        }
+        bg.nbins *= bg.nb[d];
+        bg.tile *= bg.T[d];
+    }
+    bg.nbins_frame = bg.nbins;
+    bg.nbins *= p->nframes;
+    bg.P = P;
+}
+
+size_t tile_bytes(const mi355nufft_plan* p) { return (size_t)p->bg.tile * p->cx_size(); }
+
+bool use_tiled(const mi355nufft_plan* p) { return p->method != 1 && p->binned; }
+
+int64_t max_subproblems(const mi355nufft_plan* p) { return p->bg.nbins + (p->M + p->bg.P - 1) / p->bg.P; }
+
+// Workgroups of a persistent tiled launch: one per subproblem slot, at most
+// 2048 (8 per CU on 256 CUs); workgroups stride over the subproblem list.
+unsigned tiled_grid(const mi355nufft_plan* p) {
+    const int64_t n = max_subproblems(p);
+    return (unsigned)(n < 2048 ? n : 2048);
+}
+
+BinState bin_state(const mi355nufft_plan* p) {
+    BinState bs;
+    const int64_t mc = p->bin_m_cap, bc = p->bin_b_cap, sc = p->bin_s_cap;
+    bs.bin_of = p->d_bin_i;
+    bs.rank = p->d_bin_i + mc;
+    bs.perm = p->d_bin_i + 2 * mc;
+    bs.counts = p->d_bin_b;
+    bs.bin_start = p->d_bin_b + bc;
+    bs.sub_bin = p->d_sub;
+    bs.sub_start = p->d_sub + sc;
+    bs.n_sub = p->d_sub + 2 * sc;
+    bs.frame_off = p->nframes > 1 ? p->d_frame_off : nullptr;
+    bs.nframes = p->nframes;
+    return bs;
+}
+
+template <typename Tp>
+int grow(mi355nufft_plan* p, Tp** buf, int64_t& cap, int64_t need, int64_t per) {
+    if (need <= cap && *buf) return MI355NUFFT_OK;
+    if (*buf) {
+        HIP_TRY(hipStreamSynchronize(p->stream));
+        (void)hipFree(*buf);
+        *buf = nullptr;
+        cap = 0;
+    }
+    if (hipMalloc((void**)buf, (size_t)(need * per + 1) * sizeof(Tp)) != hipSuccess)
+        return fail(MI355NUFFT_ENOMEM, "binning storage (%lld entries)", (long long)(need * per));
+    cap = need;
+    return MI355NUFFT_OK;
+}
+
+template <typename T>
+int bin_points_impl(mi355nufft_plan* p) {
+    p->binned = 0;
+    if (p->method == 1 || p->M == 0) return MI355NUFFT_OK;
+    setup_bin_geom(p);
+    if (tile_bytes(p) > p->lds_limit) {
+        if (p->method == 2 || p->nframes > 1)
+            return fail(MI355NUFFT_EINVAL, "tile of %d points (%zu bytes) exceeds the LDS budget of %zu bytes",
+                        p->bg.tile, tile_bytes(p), p->lds_limit);
+        return MI355NUFFT_OK;
+    }
+    const int64_t M = p->M;
+    const int64_t max_sub = max_subproblems(p);
+    RET_IF(grow(p, &p->d_bin_i, p->bin_m_cap, M, 3));
+    RET_IF(grow(p, &p->d_bin_b, p->bin_b_cap, (int64_t)p->bg.nbins, 2));
+    RET_IF(grow(p, &p->d_sub, p->bin_s_cap, max_sub, 2));
+    BinState bs = bin_state(p);
+    InterpArgs<T, false> ia;
+    int32_t J[3] = {p->J[0], p->J[1], p->J[2]};
+    const void* dummy[3] = {p->d_tab[0], p->d_tab[0], p->d_tab[0]};
+    fill_interp_args<T, false>(ia, p->ndim, p->K, J, p->L, dummy, p->d_tm, M, 1);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(nblocks(p->bg.nbins)), dim3(kBlock), 0, p->stream, bs.counts,
+                       (int64_t)p->bg.nbins, 0);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL((bin_count_kernel<T>), dim3(nblocks(M)), dim3(kBlock), 0, p->stream, (const T*)p->d_tm, M,
+                       p->ndim, ia.ax[0], ia.ax[1], ia.ax[2], p->bg, bs);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, p->stream, bs, p->bg.nbins, p->bg.P, (int)max_sub);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(bin_scatter_kernel, dim3(nblocks(M)), dim3(kBlock), 0, p->stream, M, bs);
+    LAUNCH_CHECK();
+    p->binned = 1;
+    return MI355NUFFT_OK;
+}
+
+int bin_points(mi355nufft_plan* p) { return p->precision ? bin_points_impl<double>(p) : bin_points_impl<float>(p); }
+
+template <typename T, bool CK, int NDIM, int JMAX>
+void launch_tiled(bool adjoint, const TiledArgs<T, CK>& a, unsigned nsub, size_t lds, hipStream_t s) {
+    if (adjoint)
+        hipLaunchKernelGGL((spread_tiled_kernel<T, NDIM, CK, JMAX>), dim3(nsub), dim3(kBlock), lds, s, a);
+    else
+        hipLaunchKernelGGL((interp_tiled_kernel<T, NDIM, CK, JMAX>), dim3(nsub), dim3(kBlock), lds, s, a);
+}
+
+template <typename T, bool CK>
+int run_tiled(mi355nufft_plan* p, bool adjoint, const void* samples_in, void* samples_out, const void* grid_in,
+              void* grid_out, int32_t nreps) {
+    TiledArgs<T, CK> a;
+    std::memset(&a, 0, sizeof(a));
+    int32_t J[3] = {p->J[0], p->J[1], p->J[2]};
+    fill_interp_args<T, CK>(a.ia, p->ndim, p->K, J, p->L, (const void* const*)p->d_tab, p->d_tm, p->M, nreps);
+    a.ia.grid = (const cx<T>*)grid_in;
+    a.ia.grid_out = (cx<T>*)grid_out;
+    a.ia.in = (const cx<T>*)samples_in;
+    a.ia.out = (cx<T>*)samples_out;
+    a.ia.phase = p->has_phase ? (const cx<T>*)p->d_phase : nullptr;
+    a.bg = p->bg;
+    a.bs = bin_state(p);
+    a.use_pb = p->phasing == 0;
+    for (int d = 0; d < 3; ++d) a.pb[d] = (T)p->phase_coef[d];
+    a.ortho = adjoint ? 0 : p->ortho;
+    a.scale_ortho = (T)p->scale_ortho;
+    const unsigned nsub = tiled_grid(p);
+    const size_t lds = tile_bytes(p);
+    const int jm = jmax_of(p->ndim, J);
+#define MI355_TCASE(ND, JM)                                                  \
+    if (p->ndim == ND && jm == JM) {                                         \
+        launch_tiled<T, CK, ND, JM>(adjoint, a, nsub, lds, p->stream);       \
+        LAUNCH_CHECK();                                                      \
+        return MI355NUFFT_OK;                                                \
+    }
+    MI355_TCASE(1, 4) MI355_TCASE(1, 8) MI355_TCASE(1, 16)
+    MI355_TCASE(2, 4) MI355_TCASE(2, 8) MI355_TCASE(2, 16)
+    MI355_TCASE(3, 4) MI355_TCASE(3, 8) MI355_TCASE(3, 16)
+#undef MI355_TCASE
+    return fail(MI355NUFFT_EINVAL, "unsupported ndim/J combination");
+}
+
+template <typename T>
+int run_tiled_any(mi355nufft_plan* p, bool adjoint, const void* samples_in, void* samples_out, const void* grid_in,
+                  void* grid_out, int32_t nreps) {
+    return p->phasing ? run_tiled<T, true>(p, adjoint, samples_in, samples_out, grid_in, grid_out, nreps)
+                      : run_tiled<T, false>(p, adjoint, samples_in, samples_out, grid_in, grid_out, nreps);
+}
+
+template <typename T, bool CK, int NDIM, int JMAX>
+void launch_grad(const TiledArgs<T, CK>& a, const GradArgs<T>& g, unsigned nsub, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL((grad_tiled_kernel<T, NDIM, CK, JMAX>), dim3(nsub), dim3(kBlock), lds, s, a, g);
+}
+
+template <typename T, bool CK>
+int run_grad(mi355nufft_plan* p, const void* grid, const void* w, void* grad, int32_t nreps) {
+    TiledArgs<T, CK> a;
+    std::memset(&a, 0, sizeof(a));
+    int32_t J[3] = {p->J[0], p->J[1], p->J[2]};
+    fill_interp_args<T, CK>(a.ia, p->ndim, p->K, J, p->L, (const void* const*)p->d_tab, p->d_tm, p->M, nreps);
+    a.ia.grid = (const cx<T>*)grid;
+    a.ia.phase = p->has_phase ? (const cx<T>*)p->d_phase : nullptr;
+    a.bg = p->bg;
+    a.bs = bin_state(p);
+    a.use_pb = p->phasing == 0;
+    for (int d = 0; d < 3; ++d) a.pb[d] = (T)p->phase_coef[d];
+    a.ortho = p->ortho;
+    a.scale_ortho = (T)p->scale_ortho;
+    GradArgs<T> g;
+    g.w = (const cx<T>*)w;
+    g.grad = (T*)grad;
+    for (int d = 0; d < 3; ++d) {
+        g.dtdw[d] = (T)((double)p->K[d] / (2.0 * M_PI));
+        g.shift[d] = (T)p->shift[d];
+    }
+    const unsigned nsub = tiled_grid(p);
+    const size_t lds = tile_bytes(p);
+    const int jm = jmax_of(p->ndim, J);
+#define MI355_GCASE(ND, JM)                                            \
+    if (p->ndim == ND && jm == JM) {                                   \
+        launch_grad<T, CK, ND, JM>(a, g, nsub, lds, p->stream);        \
+        LAUNCH_CHECK();                                                \
+        return MI355NUFFT_OK;                                          \
+    }
+    MI355_GCASE(1, 4) MI355_GCASE(1, 8) MI355_GCASE(1, 16)
+    MI355_GCASE(2, 4) MI355_GCASE(2, 8) MI355_GCASE(2, 16)
+    MI355_GCASE(3, 4) MI355_GCASE(3, 8) MI355_GCASE(3, 16)
+#undef MI355_GCASE
+    return fail(MI355NUFFT_EINVAL, "unsupported ndim/J combination");
 }
 
 int need_points(const mi355nufft_plan* p) {
@@ -452,6 +697,12 @@ int mi355nufft_plan_create(mi355nufft_plan** plan, int ndim, const int64_t* Nd, 
         p->phase_coef[d] = phase_coef ? phase_coef[d] : 0.0;
     }
     p->scale_ortho = p->ortho ? std::sqrt((double)p->nk) : 1.0;
+    {
+        int dev = 0, lds = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
+            p->lds_limit = (size_t)(lds < 65536 ? lds : 65536);
+    }
     for (int d = 0; d < ndim; ++d) {
         const size_t tab_bytes = (size_t)(Jd[d] * L + 1) * (phasing ? p->cx_size() : p->real_size());
         if (hipMalloc(&p->d_tab[d], tab_bytes) != hipSuccess ||
@@ -486,6 +737,10 @@ int mi355nufft_plan_destroy(mi355nufft_plan* p) {
     if (p->d_phase) (void)hipFree(p->d_phase);
     if (p->d_grid) (void)hipFree(p->d_grid);
     if (p->fft_work) (void)hipFree(p->fft_work);
+    if (p->d_bin_i) (void)hipFree(p->d_bin_i);
+    if (p->d_bin_b) (void)hipFree(p->d_bin_b);
+    if (p->d_sub) (void)hipFree(p->d_sub);
+    if (p->d_frame_off) (void)hipFree(p->d_frame_off);
     delete p;
     return MI355NUFFT_OK;
 }
@@ -499,8 +754,39 @@ int mi355nufft_set_stream(mi355nufft_plan* p, void* stream) {
 
 int mi355nufft_set_points(mi355nufft_plan* p, const void* omega, int omega_double, int64_t M, int64_t stride_m,
                           int64_t stride_d, const double* shift, int phase_mode, int dot_single) {
+    return mi355nufft_set_points_frames(p, omega, omega_double, M, stride_m, stride_d, shift, phase_mode, dot_single,
+                                        nullptr, 1);
+}
+
+int mi355nufft_set_points_frames(mi355nufft_plan* p, const void* omega, int omega_double, int64_t M,
+                                 int64_t stride_m, int64_t stride_d, const double* shift, int phase_mode,
+                                 int dot_single, const int64_t* frame_offsets, int32_t nframes) {
     g_err.clear();
     RET_IF(check_plan(p));
+    if (nframes < 1 || nframes > 65535) return fail(MI355NUFFT_EINVAL, "nframes must be in 1..65535");
+    if (nframes > 1) {
+        if (!frame_offsets) return fail(MI355NUFFT_EINVAL, "frame_offsets is required for nframes > 1");
+        if (frame_offsets[0] != 0 || frame_offsets[nframes] != M)
+            return fail(MI355NUFFT_EINVAL, "frame_offsets must start at 0 and end at M");
+        for (int f = 0; f < nframes; ++f)
+            if (frame_offsets[f + 1] < frame_offsets[f])
+                return fail(MI355NUFFT_EINVAL, "frame_offsets must be non-decreasing");
+        if (p->method == 1) return fail(MI355NUFFT_EINVAL, "frames need the tiled kernels (binning method 0 or 2)");
+        if (nframes + 1 > p->frame_cap) {
+            if (p->d_frame_off) {
+                HIP_TRY(hipStreamSynchronize(p->stream));
+                (void)hipFree(p->d_frame_off);
+                p->d_frame_off = nullptr;
+            }
+            if (hipMalloc((void**)&p->d_frame_off, (size_t)(nframes + 1) * sizeof(int64_t)) != hipSuccess)
+                return fail(MI355NUFFT_ENOMEM, "frame offsets");
+            p->frame_cap = nframes + 1;
+        }
+        HIP_TRY(hipStreamSynchronize(p->stream));
+        HIP_TRY(hipMemcpy(p->d_frame_off, frame_offsets, (size_t)(nframes + 1) * sizeof(int64_t),
+                          hipMemcpyHostToDevice));
+    }
+    p->nframes = nframes;
     if (M < 0 || M >= ((int64_t)1 << 31)) return fail(MI355NUFFT_EINVAL, "M must be in [0, 2^31)");
     if (M > 0 && !omega) return fail(MI355NUFFT_EINVAL, "omega is null");
     if (phase_mode < 0 || phase_mode > 2) return fail(MI355NUFFT_EINVAL, "phase_mode must be 0, 1 or 2");
@@ -525,6 +811,8 @@ int mi355nufft_set_points(mi355nufft_plan* p, const void* omega, int omega_doubl
     }
     p->M = M;
     p->has_phase = phase_mode != 0;
+    p->binned = 0;
+    for (int d = 0; d < 3; ++d) p->shift[d] = (phase_mode && d < p->ndim) ? shift[d] : 0.0;
     if (M == 0) return MI355NUFFT_OK;
     double gam[3] = {0, 0, 0};
     for (int d = 0; d < p->ndim; ++d) gam[d] = 2.0 * M_PI / (double)p->K[d];
@@ -544,6 +832,38 @@ int mi355nufft_set_points(mi355nufft_plan* p, const void* omega, int omega_doubl
     }
 #undef SP_LAUNCH
     LAUNCH_CHECK();
+    return bin_points(p);
+}
+
+int mi355nufft_set_binning(mi355nufft_plan* p, const int32_t* bin_size, int32_t max_points, int method) {
+    g_err.clear();
+    RET_IF(check_plan(p));
+    if (method < 0 || method > 2) return fail(MI355NUFFT_EINVAL, "method must be 0, 1 or 2");
+    if (max_points < 0) return fail(MI355NUFFT_EINVAL, "max_points must be >= 0");
+    for (int d = 0; d < p->ndim; ++d) {
+        const int b = bin_size ? bin_size[d] : 0;
+        if (b < 0 || b > 4096) return fail(MI355NUFFT_EINVAL, "bin_size[%d] must be in 0..4096", d);
+        p->bin_req[d] = b;
+    }
+    p->P_req = max_points;
+    p->method = method;
+    if (p->d_tm && p->M > 0) return bin_points(p);
+    if (method == 2) {
+        setup_bin_geom(p);
+        if (tile_bytes(p) > p->lds_limit)
+            return fail(MI355NUFFT_EINVAL, "tile of %d points exceeds the LDS budget", p->bg.tile);
+    }
+    return MI355NUFFT_OK;
+}
+
+int mi355nufft_plan_info(const mi355nufft_plan* p, int64_t* info, int32_t n) {
+    g_err.clear();
+    RET_IF(check_plan(p));
+    if (!info || n < 1) return fail(MI355NUFFT_EINVAL, "info buffer required");
+    const int64_t v[13] = {p->ndim, p->M, p->bg.B[0], p->bg.B[1], p->bg.B[2], p->bg.P, p->bg.nbins, p->bg.tile,
+                           use_tiled(p) ? 1 : 0, p->binned ? max_subproblems(p) : 0, p->method, (int64_t)p->lds_limit,
+                           p->nframes};
+    for (int i = 0; i < n && i < 13; ++i) info[i] = v[i];
     return MI355NUFFT_OK;
 }
 
@@ -585,6 +905,20 @@ int mi355nufft_spread(mi355nufft_plan* p, const void* y, void* grid, int32_t nre
     return p->precision ? spread_impl<double>(p, y, grid, nreps) : spread_impl<float>(p, y, grid, nreps);
 }
 
+int mi355nufft_interp_grad(mi355nufft_plan* p, const void* grid, const void* w, void* grad, int32_t nreps) {
+    g_err.clear();
+    RET_IF(check_plan(p));
+    RET_IF(check_nreps(nreps));
+    RET_IF(need_points(p));
+    if (p->M == 0) return MI355NUFFT_OK;
+    if (!grid || !w || !grad) return fail(MI355NUFFT_EINVAL, "grid, w and grad are required");
+    if (!use_tiled(p))
+        return fail(MI355NUFFT_ESTATE, "the trajectory gradient runs on the tiled kernels (binning method 0 or 2)");
+    if (p->precision)
+        return p->phasing ? run_grad<double, true>(p, grid, w, grad, nreps) : run_grad<double, false>(p, grid, w, grad, nreps);
+    return p->phasing ? run_grad<float, true>(p, grid, w, grad, nreps) : run_grad<float, false>(p, grid, w, grad, nreps);
+}
+
 int mi355nufft_crop_deapod(mi355nufft_plan* p, const void* grid, void* x, const int64_t* xs, int32_t nreps) {
     g_err.clear();
     RET_IF(check_plan(p));
@@ -600,9 +934,11 @@ int mi355nufft_forward(mi355nufft_plan* p, const void* x, const int64_t* xs, voi
     RET_IF(check_nreps(nreps));
     RET_IF(need_points(p));
     if (!x || !xs || (!y && p->M > 0)) return fail(MI355NUFFT_EINVAL, "x, xs and y are required");
-    RET_IF(get_grid(p, nreps, &grid));
-    RET_IF(mi355nufft_deapod_pad(p, x, xs, grid, nreps));
-    RET_IF(mi355nufft_fft(p, grid, nreps, -1));
+    const int32_t nimg = p->nframes * nreps;
+    RET_IF(check_nreps(nimg));
+    RET_IF(get_grid(p, nimg, &grid));
+    RET_IF(mi355nufft_deapod_pad(p, x, xs, grid, nimg));
+    RET_IF(mi355nufft_fft(p, grid, nimg, -1));
     return mi355nufft_interp(p, grid, y, nreps);
 }
 
@@ -612,10 +948,12 @@ int mi355nufft_adjoint(mi355nufft_plan* p, const void* y, void* x, const int64_t
     RET_IF(check_nreps(nreps));
     RET_IF(need_points(p));
     if (!x || !xs || (!y && p->M > 0)) return fail(MI355NUFFT_EINVAL, "y, x and xs are required");
-    RET_IF(get_grid(p, nreps, &grid));
+    const int32_t nimg = p->nframes * nreps;
+    RET_IF(check_nreps(nimg));
+    RET_IF(get_grid(p, nimg, &grid));
     RET_IF(mi355nufft_spread(p, y, grid, nreps));
-    RET_IF(mi355nufft_fft(p, grid, nreps, 1));
-    return mi355nufft_crop_deapod(p, grid, x, xs, nreps);
+    RET_IF(mi355nufft_fft(p, grid, nimg, 1));
+    return mi355nufft_crop_deapod(p, grid, x, xs, nimg);
 }
 
 int mi355nufft_table1_forward(int ndim, const int64_t* Kd, const int32_t* Jd, int32_t L, int precision,

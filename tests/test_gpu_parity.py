@@ -323,3 +323,61 @@ def test_non_finite_points_do_not_fault():
     xa = A.adj(np.ones(5))
     assert xa.shape == Nd
     torch.cuda.synchronize()
+
+
+# ------------------------------------------------ kernel families / binning
+BINNINGS = [
+    # (method, bin_size, max_points)
+    ("gm", None, 0),
+    ("tiled", None, 0),
+    ("tiled", (8, 8), 7),        # many small subproblems
+    ("tiled", (64, 16), 1000),   # anisotropic bins
+    ("tiled", (200, 40), 64),    # bins larger than the grid on axis 0 (clamped to K)
+]
+
+
+@pytest.mark.parametrize("method,bin_size,max_points", BINNINGS)
+@pytest.mark.parametrize("precision", ["single", "double"])
+@pytest.mark.parametrize("phasing", ["real", "complex"])
+def test_binning_variants_vs_oracle(method, bin_size, max_points, precision, phasing):
+    Nd, Kd = (40, 36), (80, 75)
+    om = trj.golden_angle_2d(50, 80)
+    A, B = _pair(Nd, om, Jd=(6, 5), Kd=Kd, Ld=1024, precision=precision, phasing=phasing, ortho=True,
+                 n_shift=(7.5, -3.0))
+    A.set_binning(bin_size, max_points, method)
+    info = A.plan_info()
+    assert info["tiled"] == (method == "tiled")
+    x = trj.complex_gaussian(Nd + (2,), seed=21)
+    y = A.fft(x)
+    assert rel_l2(y, B.fft(x)) <= TOL[precision]
+    assert rel_l2(A.adj(y), B.adj(y)) <= TOL[precision]
+
+
+@pytest.mark.parametrize("ndim", [1, 3])
+def test_tiled_1d_3d_small_grids(ndim):
+    """Tiles wider than the grid (K < B + J - 1) wrap onto themselves."""
+    if ndim == 1:
+        Nd, Kd, om = (10,), (20,), np.random.default_rng(0).uniform(-np.pi, np.pi, (300, 1))
+    else:
+        Nd, Kd, om = (6, 8, 5), (12, 16, 10), trj.kooshball_3d(40, 16, seed=2)
+    for precision in ("single", "double"):
+        A, B = _pair(Nd, om, Jd=6, Kd=Kd, Ld=512, precision=precision)
+        A.set_binning(None, 5, "tiled")
+        x = trj.complex_gaussian(Nd, seed=22)
+        y = A.fft(x)
+        assert rel_l2(y, B.fft(x)) <= TOL[precision]
+        assert rel_l2(A.adj(y), B.adj(y)) <= TOL[precision]
+
+
+def test_default_binning_active():
+    A = NufftBase((64, 64), trj.radial_2d(16, 128), Jd=6, Kd=(128, 128), device=dev())
+    info = A.plan_info()
+    assert info["tiled"] and info["bin_size"][:2] == (32, 32)
+    assert info["nbins"] == 16 and info["tile"] == 37 * 37
+
+
+def test_tiled_rejects_tile_over_lds_budget():
+    """A forced tiled method with a tile beyond the LDS budget is an error."""
+    A = NufftBase((40, 36), trj.golden_angle_2d(10, 80), Jd=6, Kd=(80, 75), precision="double", device=dev())
+    with pytest.raises(ValueError, match="LDS"):
+        A.set_binning((200, 200), 64, "tiled")
