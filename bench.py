@@ -37,6 +37,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the complex128 / 3-D extra measurements")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--eager", action="store_true", help="time eager launches instead of a HIP-graph replay")
     return ap.parse_args()
 
 
@@ -100,6 +101,23 @@ class GpuRun:
         chk(lib.mi355nufft_forward(p, self.px, self.xs, self.py, 1, self.pg))
         chk(lib.mi355nufft_adjoint(p, self.py, self.pxo, self.xos, 1, self.pg))
 
+    def capture(self):
+        """Record one forward + adjoint into a HIP graph (torch CUDAGraph API)."""
+        torch = self.torch
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            self.A._sync_stream()
+            self.pair()
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.A._sync_stream()
+            self.pair()
+        self.A._sync_stream()
+        return self.graph.replay
+
     def stages(self):
         lib, p = self.lib, self.p
         return {
@@ -134,6 +152,101 @@ class GpuRun:
                     self.L.check(rc)
             out[name] = self.time(call, steps, warmup)
         return out
+
+
+def run_c4(steps=3, autotune=True):
+    """24 frames x 32 coils, golden-angle frames with 13..89 spokes x 512,
+    Nd 256^2, Kd 512^2, J 6, complex64: one graph-captured frames plan vs a
+    per-frame loop of plain operators (32 coils each)."""
+    import torch
+    from mrrt.nufft_amd import NufftBase
+    from mrrt.nufft_amd import trajectories as trj
+    from mrrt.nufft_amd.batched import NufftFrames
+    dev = torch.device("cuda", 0)
+    F, C, Nd, Kd = 24, 32, (256, 256), (512, 512)
+    frames = trj.golden_angle_frames(F, 512, 13, 89, seed=0)
+    B = NufftFrames(Nd, frames, C, Jd=6, Kd=Kd, Ld=1024, device=dev)
+    tuned = None
+    if autotune:
+        from mrrt.nufft_amd import autotune as at
+        b, pmax, _ = at.tune(B.op, nreps=C, use_cache=False)
+        tuned = {"bin_size": b, "max_points": pmax}
+    rng = torch.Generator(device=dev).manual_seed(0)
+    x = torch.randn((F, C) + Nd, dtype=torch.complex64, device=dev, generator=rng)
+    y = torch.empty((C, B.M), dtype=torch.complex64, device=dev)
+    xa = torch.empty_like(x)
+    B.capture(x, y, xa)
+
+    def batched():
+        B.replay_forward()
+        B.replay_adjoint()
+
+    ops = [NufftBase(Nd, om, Jd=6, Kd=Kd, Ld=1024, device=dev, order="C") for om in frames]
+
+    def loop():
+        for f, op in enumerate(ops):
+            xv = x[f].permute(1, 2, 0)
+            yf = op._forward_dev(xv, C)
+            op._adjoint_dev(yf, C)
+
+    t_b = _time_fn(batched, steps, 1)
+    t_l = _time_fn(loop, steps, 1)
+    b_bytes = sum(interp_bytes(om.shape[0], 2, 512 * 512, C, 4) for om in frames)
+    return {"c4_frames_x_coils": [F, C], "c4_M_total": int(B.M),
+            "c4_batched_graph_ms_per_pair": round(t_b, 3), "c4_per_frame_loop_ms_per_pair": round(t_l, 3),
+            "c4_speedup_batched_vs_loop": round(t_l / t_b, 3),
+            "c4_transforms_per_s_batched": round(2 * F * C * 1000.0 / t_b, 1),
+            "c4_interp_bytes_per_transform": int(b_bytes), "c4_autotuned": tuned}
+
+
+def run_c5(steps=5):
+    """Differentiable spiral: 16 x 4096 samples, 8 coils, Nd 256^2, c64;
+    forward + backward (grad x and grad omega) vs plain forward + adjoint."""
+    import torch
+    from mrrt.nufft_amd import NufftBase
+    from mrrt.nufft_amd import trajectories as trj
+    from mrrt.nufft_amd.autograd import nufft
+    dev = torch.device("cuda", 0)
+    Nd, Kd, C = (256, 256), (512, 512), 8
+    om = trj.spiral_2d(16, 4096, seed=0)
+    op = NufftBase(Nd, om, Jd=6, Kd=Kd, Ld=1024, device=dev)
+    rng = torch.Generator(device=dev).manual_seed(1)
+    x = torch.randn((C,) + Nd, dtype=torch.complex64, device=dev, generator=rng).requires_grad_(True)
+    omega = torch.from_numpy(om.astype(np.float32)).to(dev).requires_grad_(True)
+    y_t = torch.randn((C, om.shape[0]), dtype=torch.complex64, device=dev, generator=rng)
+
+    def ad_step():
+        loss = ((nufft(op, x, omega) - y_t).abs() ** 2).sum()
+        gx, gw = torch.autograd.grad(loss, (x, omega))
+        return gx, gw
+
+    xv = x.detach()
+
+    def plain():
+        y = op._forward_dev(xv.permute(1, 2, 0), C)
+        op._adjoint_dev(y, C)
+
+    t_ad = _time_fn(ad_step, steps, 2)
+    t_pl = _time_fn(plain, steps, 2)
+    return {"c5_autograd_fwd_bwd_ms": round(t_ad, 4), "c5_plain_fwd_adj_ms": round(t_pl, 4),
+            "c5_autograd_overhead": round(t_ad / t_pl, 3)}
+
+
+def _time_fn(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    s.record()
+    for _ in range(steps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1000.0 / steps
+    return max(s.elapsed_time(e) / steps, wall)
 
 
 def cpu_baseline(seconds):
@@ -177,21 +290,9 @@ def main():
     run = GpuRun(wl)
     print("bench: plan ready (M=%d); timing %d steps after %d warmup" % (run.M, args.steps, args.warmup),
           file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        run.pair()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    s = torch.cuda.Event(enable_timing=True)
-    e = torch.cuda.Event(enable_timing=True)
-    s.record()
-    for _ in range(args.steps):
-        run.pair()
-    e.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = s.elapsed_time(e) / args.steps
-    ms_wall = 1000 * wall / args.steps
-    ms = max(ms_per_step, ms_wall)
+    ms_eager = _time_fn(run.pair, args.steps, args.warmup)
+    step = run.capture() if not args.eager else run.pair
+    ms = _time_fn(step, args.steps, args.warmup)
     stages = run.stage_times(max(10, args.steps // 2), 3)
     print("bench: stages (ms) %s" % json.dumps({k: round(v, 5) for k, v in stages.items()}),
           file=sys.stderr, flush=True)
@@ -216,6 +317,12 @@ def main():
         extra["c3_interp_GBps"] = round(b3 / (st3["interp"] * 1e-3) / 1e9, 1)
         extra["c3_spread_GBps"] = round(b3 / (st3["spread"] * 1e-3) / 1e9, 1)
         del r3
+        torch.cuda.empty_cache()
+        print("bench: frames batch (24 x 32)", file=sys.stderr, flush=True)
+        extra.update(run_c4())
+        torch.cuda.empty_cache()
+        print("bench: autograd spiral", file=sys.stderr, flush=True)
+        extra.update(run_c5())
     cpu = None
     if not args.no_cpu_baseline:
         print("bench: cpu baseline (~%.0f s)" % args.cpu_seconds, file=sys.stderr, flush=True)
@@ -239,6 +346,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "stages_ms": {k: round(v, 5) for k, v in stages.items()},
+        "step": "eager launches" if args.eager else "HIP graph replay of forward+adjoint",
+        "ms_per_step_eager": round(ms_eager, 5),
         "extra": extra,
     }
     print(json.dumps(line), flush=True)

@@ -124,7 +124,8 @@ int64_t mi355nufft_num_points(const mi355nufft_plan *plan);
  * Bin geometry and kernel choice for the interpolation stages (no counterpart
  * in the reference, whose kernels run one thread per sample unsorted).
  *   bin_size    ndim bin sizes in grid points (NULL or 0 entries: defaults
- *               2-D 32x32, 3-D 16x16x8 / 16x8x8 for complex128, 1-D 512)
+ *               2-D 32x32, 3-D 16x8x8, 1-D 512; the scatter tile is
+ *               accumulated in complex float64 and must fit the LDS budget)
  *   max_points  points per subproblem (0: default 256 in 2-D, 512 else)
  *   method      0 auto (tiled when the tile fits in LDS), 1 global-memory
  *               kernels on unsorted points, 2 tiled (error if it cannot)

@@ -322,6 +322,19 @@ class NufftBase(object):
         return dict(ndim=v[0], M=v[1], bin_size=tuple(v[2:5]), max_points=v[5], nbins=v[6], tile=v[7],
                     tiled=bool(v[8]), subproblem_slots=v[9], method=v[10], lds_limit=v[11], nframes=v[12])
 
+    def _h_dev(self):
+        if getattr(self, "_h_dev_cache", None) is None:
+            self._h_dev_cache = [torch.from_numpy(t).to(self.device) for t in self._tables_host]
+        return self._h_dev_cache
+
+    def _tm_dev(self):
+        """tm as (ndim, M) contiguous, the layout of the raw table1 kernels."""
+        key = self._points_version
+        if getattr(self, "_tm_dev_key", None) != key:
+            self._tm_dev_cache = self.tm.t().contiguous()
+            self._tm_dev_key = key
+        return self._tm_dev_cache
+
     def _sync_stream(self):
         s = torch.cuda.current_stream(self.device).cuda_stream
         if s != self._stream:
@@ -427,8 +440,57 @@ def _to_output(t, was_numpy):
     return t
 
 
-def nufft_forward(obj, x):
-    """Forward table1 NUFFT (mrrt/nufft/_nufft.py:1275-1397)."""
+def _raw_table1(obj, fwd, data, nreps):
+    """Raw table1 gather (fwd) or scatter on the plan's tables and points."""
+    h = obj._h_dev()
+    hp = (ctypes.c_void_p * obj.ndim)(*[t.data_ptr() for t in h])
+    tm = obj._tm_dev()
+    obj._sync_stream()
+    stream = ctypes.c_void_p(obj._stream)
+    prec = 0 if obj._precision == "single" else 1
+    ck = 1 if obj._phasing == "complex" else 0
+    geom = (obj.ndim, _lib.i64arr(obj.Kd), _lib.i32arr(obj.Jd), obj.Ld, prec, ck)
+    if fwd:
+        out = torch.empty((nreps, obj.M), dtype=obj._tcdt, device=obj.device)
+        _lib.check(obj._lib.mi355nufft_table1_forward(*geom, ctypes.c_void_p(data.data_ptr()), hp,
+                                                      ctypes.c_void_p(tm.data_ptr()), obj.M,
+                                                      ctypes.c_void_p(out.data_ptr()), nreps, stream))
+    else:
+        out = torch.empty((nreps, int(np.prod(obj.Kd))), dtype=obj._tcdt, device=obj.device)
+        _lib.check(obj._lib.mi355nufft_table1_adjoint(*geom, ctypes.c_void_p(data.data_ptr()), hp,
+                                                      ctypes.c_void_p(tm.data_ptr()), obj.M,
+                                                      ctypes.c_void_p(out.data_ptr()), nreps, stream))
+    return out
+
+
+def _phase_shift_dev(obj):
+    """Complex-kernel phase_shift exp(i omega_rdt . n_shift), complex128 (_nufft.py:898-901)."""
+    if obj._phasing != "complex" or not any(s != 0 for s in obj.n_shift):
+        return None
+    sh = torch.tensor(obj.n_shift, dtype=torch.float64, device=obj.device)
+    return torch.exp(1j * (obj._omega_rdt.double() @ sh))
+
+
+def nufft_forward(obj, x, grid_only=False):
+    """Forward table1 NUFFT (mrrt/nufft/_nufft.py:1275-1397).
+
+    ``grid_only``: ``x`` is the oversampled grid (prod(Kd), reps); only the
+    table interpolation is applied (no FFT, phase_before, ortho or
+    phase_after; the complex kernel's phase_shift is kept, :1086-1095) and
+    the (M, reps) result is returned (:1302-1304, :1376-1388).
+    """
+    if grid_only:
+        t, was_numpy = _to_device(obj, x)
+        nk = int(np.prod(obj.Kd))
+        if t.numel() % nk != 0:
+            raise ValueError("Input signal has the wrong size.")
+        nreps = t.numel() // nk
+        g = _f_reshape(t, (nk, nreps)).t().contiguous()
+        y = _raw_table1(obj, True, g, nreps)
+        ph = _phase_shift_dev(obj)
+        if ph is not None:
+            y = (y.to(torch.complex128) * ph[None, :]).to(obj._tcdt)
+        return _to_output(y.t(), was_numpy)
     t, was_numpy = _to_device(obj, x)
     nn = obj.nargin1
     if t.numel() % nn != 0:
@@ -442,13 +504,30 @@ def nufft_forward(obj, x):
     return _to_output(out, was_numpy)
 
 
-def nufft_adj(obj, xk):
-    """Adjoint table1 NUFFT (mrrt/nufft/_nufft.py:1459-1578)."""
+def nufft_adj(obj, xk, return_psf=False, grid_only=False):
+    """Adjoint table1 NUFFT (mrrt/nufft/_nufft.py:1459-1578).
+
+    ``grid_only``: return the raw gridded samples (prod(Kd), reps) before
+    conj(phase_before) and the FFT (:1507-1509).  ``return_psf``: skip
+    conj(phase_after) and return the first repetition's grid with shape Kd
+    (:1495, :1517-1518), e.g. for a Toeplitz point-spread function.
+    """
     t, was_numpy = _to_device(obj, xk)
     if t.numel() % obj.M != 0:
         raise ValueError("invalid size")
     nreps = t.numel() // obj.M
     y = _f_reshape(t, (obj.M, nreps)).t().contiguous()   # (nreps, M)
+    if grid_only or return_psf:
+        if obj._phasing == "real" and not return_psf:
+            y = y * torch.conj(obj.phase_after)[None, :]
+        ph = _phase_shift_dev(obj)
+        if ph is not None:
+            y = (y.to(torch.complex128) * torch.conj(ph)[None, :]).to(obj._tcdt)
+        g = _raw_table1(obj, False, y.contiguous(), nreps)
+        if return_psf:
+            g0 = g[0].reshape(tuple(reversed(obj.Kd))).permute(*reversed(range(obj.ndim)))
+            return _to_output(g0, was_numpy)
+        return _to_output(g.t(), was_numpy)
     out = obj._adjoint_dev(y, nreps)
     if nreps == 1:
         out = out[..., 0]

@@ -70,15 +70,16 @@ __device__ __forceinline__ int pos_mod(long long k, int K) {
 
 // Coefficients of the J taps along one axis, real kernel.  Returns the grid
 // index of the first tap modulo K.
-template <typename T, int JMAX>
+template <typename T, int JMAX, bool FIXED = false>
 __device__ __forceinline__ int axis_coefs(T t, int L, const AxisGeom& g,
                                           const T* __restrict__ hc,
                                           T (&c)[JMAX]) {
-    const long long k = first_tap(t, g.J);
+    const int J = FIXED ? JMAX : g.J;
+    const long long k = first_tap(t, J);
     const T tL = (T)L;
 #pragma unroll
     for (int j = 0; j < JMAX; ++j) {
-        if (j < g.J) {
+        if (FIXED || j < g.J) {
             const T p = (t - (T)(k + j)) * tL;
             const T pc = fmin_t(fmax_t(p, (T)g.nlo), (T)g.nhi);
             const int n = (int)floor_t(pc);
@@ -93,15 +94,16 @@ __device__ __forceinline__ int axis_coefs(T t, int L, const AxisGeom& g,
 
 // Complex-kernel variant (phasing='complex'): coefficient real and imaginary
 // parts are interpolated separately (nufft_table.template.c:670-672).
-template <typename T, int JMAX>
+template <typename T, int JMAX, bool FIXED = false>
 __device__ __forceinline__ int axis_coefs(T t, int L, const AxisGeom& g,
                                           const cx<T>* __restrict__ hc,
                                           T (&cr)[JMAX], T (&ci)[JMAX]) {
-    const long long k = first_tap(t, g.J);
+    const int J = FIXED ? JMAX : g.J;
+    const long long k = first_tap(t, J);
     const T tL = (T)L;
 #pragma unroll
     for (int j = 0; j < JMAX; ++j) {
-        if (j < g.J) {
+        if (FIXED || j < g.J) {
             const T p = (t - (T)(k + j)) * tL;
             const T pc = fmin_t(fmax_t(p, (T)g.nlo), (T)g.nhi);
             const int n = (int)floor_t(pc);
@@ -122,14 +124,15 @@ __device__ __forceinline__ int axis_coefs(T t, int L, const AxisGeom& g,
 // piecewise-linear table interpolant itself, L * (h[n+1] - h[n]) on the
 // cell used by the coefficient, so it is exact for the operator as
 // evaluated (finite differences agree away from cell boundaries).
-template <typename T, int JMAX>
+template <typename T, int JMAX, bool FIXED = false>
 __device__ __forceinline__ int axis_coefs_d(T t, int L, const AxisGeom& g, const T* __restrict__ hc,
                                             T (&c)[JMAX], T (&dc)[JMAX]) {
-    const long long k = first_tap(t, g.J);
+    const int J = FIXED ? JMAX : g.J;
+    const long long k = first_tap(t, J);
     const T tL = (T)L;
 #pragma unroll
     for (int j = 0; j < JMAX; ++j) {
-        if (j < g.J) {
+        if (FIXED || j < g.J) {
             const T p = (t - (T)(k + j)) * tL;
             const T pc = fmin_t(fmax_t(p, (T)g.nlo), (T)g.nhi);
             const int n = (int)floor_t(pc);
@@ -145,14 +148,15 @@ __device__ __forceinline__ int axis_coefs_d(T t, int L, const AxisGeom& g, const
     return pos_mod(k, g.K);
 }
 
-template <typename T, int JMAX>
+template <typename T, int JMAX, bool FIXED = false>
 __device__ __forceinline__ int axis_coefs_d(T t, int L, const AxisGeom& g, const cx<T>* __restrict__ hc,
                                             T (&cr)[JMAX], T (&ci)[JMAX], T (&dcr)[JMAX], T (&dci)[JMAX]) {
-    const long long k = first_tap(t, g.J);
+    const int J = FIXED ? JMAX : g.J;
+    const long long k = first_tap(t, J);
     const T tL = (T)L;
 #pragma unroll
     for (int j = 0; j < JMAX; ++j) {
-        if (j < g.J) {
+        if (FIXED || j < g.J) {
             const T p = (t - (T)(k + j)) * tL;
             const T pc = fmin_t(fmax_t(p, (T)g.nlo), (T)g.nhi);
             const int n = (int)floor_t(pc);
@@ -169,6 +173,9 @@ __device__ __forceinline__ int axis_coefs_d(T t, int L, const AxisGeom& g, const
     }
     return pos_mod(k, g.K);
 }
+
+// FIXED: every axis has exactly JMAX taps (compile-time trip counts, no
+// per-tap guards); otherwise J <= JMAX per axis at run time.
 
 // Grid indices of the J taps along one axis (periodic wrap).
 template <int JMAX>
@@ -190,6 +197,21 @@ __device__ __forceinline__ cx<T> cis(T phi) {
     double s, c;
     sincos((double)phi, &s, &c);
     return mkcx<T>((T)c, (T)s);
+}
+
+// Faster variant for the per-grid-point phases of the tiled kernels: the
+// float path uses the device sincosf (within 2 ulp after full range
+// reduction, like numpy's complex64 exp of the same float32 argument);
+// the double path is the double sincos.
+__device__ __forceinline__ float2 cis_tile(float phi) {
+    float s, c;
+    sincosf(phi, &s, &c);
+    return make_float2(c, s);
+}
+__device__ __forceinline__ double2 cis_tile(double phi) {
+    double s, c;
+    sincos(phi, &s, &c);
+    return make_double2(c, s);
 }
 
 }  // namespace mi355

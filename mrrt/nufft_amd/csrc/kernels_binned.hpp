@@ -183,27 +183,44 @@ template <typename T, int NDIM, bool CK>
 __device__ __forceinline__ void stage_tile(const TiledArgs<T, CK>& a, const Sub& s, const AxisGeom* ax,
                                            const cx<T>* __restrict__ g, cx<T>* tile) {
     const BinGeom& bg = a.bg;
-    for (int e = threadIdx.x; e < bg.tile; e += blockDim.x) {
-        int g0, g1, g2;
-        const int64_t gi = tile_to_grid(e, bg, s, ax, g0, g1, g2);
-        cx<T> v = g[gi];
-        if (a.use_pb) {
-            const T c[3] = {a.pb[0], a.pb[1], a.pb[2]};
-            const cx<T> p = phase_before_at<T>(c, NDIM, g0, g1, g2);
-            const T vr = v.x * p.x - v.y * p.y;
-            const T vi = v.x * p.y + v.y * p.x;
-            v.x = vr;
-            v.y = vi;
+    constexpr int U = 4;  // grid loads in flight per thread before the phase math
+    const T c[3] = {a.pb[0], a.pb[1], a.pb[2]};
+    for (int e0 = threadIdx.x; e0 < bg.tile; e0 += U * blockDim.x) {
+        cx<T> v[U];
+        T phi[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = e0 + u * blockDim.x;
+            if (e < bg.tile) {
+                int g0, g1, g2;
+                const int64_t gi = tile_to_grid(e, bg, s, ax, g0, g1, g2);
+                v[u] = g[gi];
+                phi[u] = phase_before_arg<T>(c, NDIM, g0, g1, g2);
+            }
         }
-        if (a.ortho) {
-            v.x = v.x / a.scale_ortho;
-            v.y = v.y / a.scale_ortho;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = e0 + u * blockDim.x;
+            if (e < bg.tile) {
+                cx<T> w = v[u];
+                if (a.use_pb) {
+                    const cx<T> p = cis_tile(phi[u]);
+                    const T vr = w.x * p.x - w.y * p.y;
+                    const T vi = w.x * p.y + w.y * p.x;
+                    w.x = vr;
+                    w.y = vi;
+                }
+                if (a.ortho) {
+                    w.x = w.x / a.scale_ortho;
+                    w.y = w.y / a.scale_ortho;
+                }
+                tile[e] = w;
+            }
         }
-        tile[e] = v;
     }
 }
 
-template <typename T, int NDIM, bool CK, int JMAX>
+template <typename T, int NDIM, bool CK, int JMAX, bool FIXED>
 __global__ __launch_bounds__(256) void interp_tiled_kernel(TiledArgs<T, CK> a) {
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* tile = reinterpret_cast<cx<T>*>(smem);
@@ -227,44 +244,45 @@ __global__ __launch_bounds__(256) void interp_tiled_kernel(TiledArgs<T, CK> a) {
             for (int d = 0; d < NDIM; ++d) {
                 const T t = ia.tm[(int64_t)d * ia.M + m];
                 int k0;
-                if constexpr (CK) k0 = axis_coefs<T, JMAX>(t, ia.L, ax[d], ia.h[d], cr[d], ci[d]);
-                else k0 = axis_coefs<T, JMAX>(t, ia.L, ax[d], ia.h[d], cr[d]);
+                if constexpr (CK) k0 = axis_coefs<T, JMAX, FIXED>(t, ia.L, ax[d], ia.h[d], cr[d], ci[d]);
+                else k0 = axis_coefs<T, JMAX, FIXED>(t, ia.L, ax[d], ia.h[d], cr[d]);
                 l[d] = k0 - s.o[d];
             }
             T sr = 0, si = 0;
             if constexpr (NDIM == 1) {
 #pragma unroll
                 for (int j1 = 0; j1 < JMAX; ++j1)
-                    if (j1 < ax[0].J) {
+                    if (FIXED || j1 < ax[0].J) {
                         const cx<T> v = tile[l[0] + j1];
                         madd<T, CK>(sr, si, cr[0][j1], ci[0][j1], v.x, v.y);
                     }
             } else if constexpr (NDIM == 2) {
 #pragma unroll
                 for (int j2 = 0; j2 < JMAX; ++j2)
-                    if (j2 < ax[1].J) {
+                    if (FIXED || j2 < ax[1].J) {
                         const cx<T>* row = tile + (l[1] + j2) * T0 + l[0];
                         T s1r = 0, s1i = 0;
 #pragma unroll
                         for (int j1 = 0; j1 < JMAX; ++j1)
-                            if (j1 < ax[0].J) {
+                            if (FIXED || j1 < ax[0].J) {
                                 const cx<T> v = row[j1];
                                 madd<T, CK>(s1r, s1i, cr[0][j1], ci[0][j1], v.x, v.y);
                             }
                         madd<T, CK>(sr, si, cr[1][j2], ci[1][j2], s1r, s1i);
                     }
             } else {
+#pragma unroll(FIXED ? JMAX : 1)
                 for (int j3 = 0; j3 < JMAX; ++j3)
-                    if (j3 < ax[2].J) {
+                    if (FIXED || j3 < ax[2].J) {
                         T s2r = 0, s2i = 0;
 #pragma unroll
                         for (int j2 = 0; j2 < JMAX; ++j2)
-                            if (j2 < ax[1].J) {
+                            if (FIXED || j2 < ax[1].J) {
                                 const cx<T>* row = tile + ((l[2] + j3) * T1 + (l[1] + j2)) * T0 + l[0];
                                 T s1r = 0, s1i = 0;
 #pragma unroll
                                 for (int j1 = 0; j1 < JMAX; ++j1)
-                                    if (j1 < ax[0].J) {
+                                    if (FIXED || j1 < ax[0].J) {
                                         const cx<T> v = row[j1];
                                         madd<T, CK>(s1r, s1i, cr[0][j1], ci[0][j1], v.x, v.y);
                                     }
@@ -289,13 +307,17 @@ __global__ __launch_bounds__(256) void interp_tiled_kernel(TiledArgs<T, CK> a) {
     }
 }
 
-template <typename T, int NDIM, bool CK, int JMAX>
+template <typename T, int NDIM, bool CK, int JMAX, bool FIXED>
 __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const InterpArgs<T, CK>& ia = a.ia;
     const BinGeom& bg = a.bg;
-    T* tre = reinterpret_cast<T*>(smem);
-    T* tim = tre + bg.tile;
+    // The tile accumulates in float64 for both precisions: on gfx950 an LDS
+    // float32 atomic add occupies the LDS for ~8x the cycles of a float64 one
+    // (measured: SQ_LDS_IDX_ACTIVE per ds_add_f32 vs ds_add_f64), and the
+    // wider sum is also more accurate than the reference's float32 sum.
+    double* tre = reinterpret_cast<double*>(smem);
+    double* tim = tre + bg.tile;
     const AxisGeom ax[3] = {ia.ax[0], ia.ax[1], ia.ax[2]};
     const int T0 = bg.T[0], T1 = bg.T[1];
     const int tid = threadIdx.x;
@@ -305,8 +327,8 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
     load_sub(bg, a.bs, NDIM, sid, s);
     for (int r = 0; r < ia.nreps; ++r) {
         for (int e = tid; e < bg.tile; e += blockDim.x) {
-            tre[e] = (T)0;
-            tim[e] = (T)0;
+            tre[e] = 0.0;
+            tim[e] = 0.0;
         }
         __syncthreads();
         for (int i = s.start + tid; i < s.end; i += blockDim.x) {
@@ -317,8 +339,8 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
             for (int d = 0; d < NDIM; ++d) {
                 const T t = ia.tm[(int64_t)d * ia.M + m];
                 int k0;
-                if constexpr (CK) k0 = axis_coefs<T, JMAX>(t, ia.L, ax[d], ia.h[d], cr[d], ci[d]);
-                else k0 = axis_coefs<T, JMAX>(t, ia.L, ax[d], ia.h[d], cr[d]);
+                if constexpr (CK) k0 = axis_coefs<T, JMAX, FIXED>(t, ia.L, ax[d], ia.h[d], cr[d], ci[d]);
+                else k0 = axis_coefs<T, JMAX, FIXED>(t, ia.L, ax[d], ia.h[d], cr[d]);
                 l[d] = k0 - s.o[d];
             }
             cx<T> f = ia.in[(int64_t)r * ia.M + m];
@@ -332,46 +354,47 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
             if constexpr (NDIM == 1) {
 #pragma unroll
                 for (int j1 = 0; j1 < JMAX; ++j1)
-                    if (j1 < ax[0].J) {
+                    if (FIXED || j1 < ax[0].J) {
                         T vr, vi;
                         cmul_conj<T, CK>(vr, vi, cr[0][j1], ci[0][j1], f.x, f.y);
-                        atomicAdd(tre + l[0] + j1, vr);
-                        atomicAdd(tim + l[0] + j1, vi);
+                        atomicAdd(tre + l[0] + j1, (double)vr);
+                        atomicAdd(tim + l[0] + j1, (double)vi);
                     }
             } else if constexpr (NDIM == 2) {
 #pragma unroll
                 for (int j2 = 0; j2 < JMAX; ++j2)
-                    if (j2 < ax[1].J) {
+                    if (FIXED || j2 < ax[1].J) {
                         T v2r, v2i;
                         cmul_conj<T, CK>(v2r, v2i, cr[1][j2], ci[1][j2], f.x, f.y);
                         const int row = (l[1] + j2) * T0 + l[0];
 #pragma unroll
                         for (int j1 = 0; j1 < JMAX; ++j1)
-                            if (j1 < ax[0].J) {
+                            if (FIXED || j1 < ax[0].J) {
                                 T vr, vi;
                                 cmul_conj<T, CK>(vr, vi, cr[0][j1], ci[0][j1], v2r, v2i);
-                                atomicAdd(tre + row + j1, vr);
-                                atomicAdd(tim + row + j1, vi);
+                                atomicAdd(tre + row + j1, (double)vr);
+                                atomicAdd(tim + row + j1, (double)vi);
                             }
                     }
             } else {
+#pragma unroll(FIXED ? JMAX : 1)
                 for (int j3 = 0; j3 < JMAX; ++j3)
-                    if (j3 < ax[2].J) {
+                    if (FIXED || j3 < ax[2].J) {
                         T v3r, v3i;
                         cmul_conj<T, CK>(v3r, v3i, cr[2][j3], ci[2][j3], f.x, f.y);
 #pragma unroll
                         for (int j2 = 0; j2 < JMAX; ++j2)
-                            if (j2 < ax[1].J) {
+                            if (FIXED || j2 < ax[1].J) {
                                 T v2r, v2i;
                                 cmul_conj<T, CK>(v2r, v2i, cr[1][j2], ci[1][j2], v3r, v3i);
                                 const int row = ((l[2] + j3) * T1 + (l[1] + j2)) * T0 + l[0];
 #pragma unroll
                                 for (int j1 = 0; j1 < JMAX; ++j1)
-                                    if (j1 < ax[0].J) {
+                                    if (FIXED || j1 < ax[0].J) {
                                         T vr, vi;
                                         cmul_conj<T, CK>(vr, vi, cr[0][j1], ci[0][j1], v2r, v2i);
-                                        atomicAdd(tre + row + j1, vr);
-                                        atomicAdd(tim + row + j1, vi);
+                                        atomicAdd(tre + row + j1, (double)vr);
+                                        atomicAdd(tim + row + j1, (double)vi);
                                     }
                             }
                     }
@@ -380,13 +403,14 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
         __syncthreads();
         cx<T>* __restrict__ g = ia.grid_out + ((int64_t)s.frame * ia.nreps + r) * ia.kstride;
         for (int e = tid; e < bg.tile; e += blockDim.x) {
-            T vr = tre[e], vi = tim[e];
-            if (vr == (T)0 && vi == (T)0) continue;
+            const double ar = tre[e], ai = tim[e];
+            if (ar == 0.0 && ai == 0.0) continue;
+            T vr = (T)ar, vi = (T)ai;
             int g0, g1, g2;
             const int64_t gi = tile_to_grid(e, bg, s, ax, g0, g1, g2);
             if (a.use_pb) {  // * conj(phase_before)
                 const T c[3] = {a.pb[0], a.pb[1], a.pb[2]};
-                const cx<T> p = phase_before_at<T>(c, NDIM, g0, g1, g2);
+                const cx<T> p = cis_tile(phase_before_arg<T>(c, NDIM, g0, g1, g2));
                 const T wr = vr * p.x + vi * p.y;
                 const T wi = vi * p.x - vr * p.y;
                 vr = wr;
@@ -420,7 +444,7 @@ __device__ __forceinline__ void cmadd(T& ar, T& ai, T cr, T ci, T vr, T vi) {
     madd<T, CK>(ar, ai, cr, ci, vr, vi);
 }
 
-template <typename T, int NDIM, bool CK, int JMAX>
+template <typename T, int NDIM, bool CK, int JMAX, bool FIXED>
 __global__ __launch_bounds__(256) void grad_tiled_kernel(TiledArgs<T, CK> a, GradArgs<T> gr) {
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* tile = reinterpret_cast<cx<T>*>(smem);
@@ -444,8 +468,8 @@ __global__ __launch_bounds__(256) void grad_tiled_kernel(TiledArgs<T, CK> a, Gra
             for (int d = 0; d < NDIM; ++d) {
                 const T t = ia.tm[(int64_t)d * ia.M + m];
                 int k0;
-                if constexpr (CK) k0 = axis_coefs_d<T, JMAX>(t, ia.L, ax[d], ia.h[d], cr[d], ci[d], dr[d], di[d]);
-                else k0 = axis_coefs_d<T, JMAX>(t, ia.L, ax[d], ia.h[d], cr[d], dr[d]);
+                if constexpr (CK) k0 = axis_coefs_d<T, JMAX, FIXED>(t, ia.L, ax[d], ia.h[d], cr[d], ci[d], dr[d], di[d]);
+                else k0 = axis_coefs_d<T, JMAX, FIXED>(t, ia.L, ax[d], ia.h[d], cr[d], dr[d]);
                 l[d] = k0 - s.o[d];
             }
             // S[0] = value, S[1 + d] = derivative along axis d
@@ -453,7 +477,7 @@ __global__ __launch_bounds__(256) void grad_tiled_kernel(TiledArgs<T, CK> a, Gra
             if constexpr (NDIM == 1) {
 #pragma unroll
                 for (int j1 = 0; j1 < JMAX; ++j1)
-                    if (j1 < ax[0].J) {
+                    if (FIXED || j1 < ax[0].J) {
                         const cx<T> v = tile[l[0] + j1];
                         cmadd<T, CK>(Sr[0], Si[0], cr[0][j1], ci[0][j1], v.x, v.y);
                         cmadd<T, CK>(Sr[1], Si[1], dr[0][j1], di[0][j1], v.x, v.y);
@@ -461,12 +485,12 @@ __global__ __launch_bounds__(256) void grad_tiled_kernel(TiledArgs<T, CK> a, Gra
             } else if constexpr (NDIM == 2) {
 #pragma unroll
                 for (int j2 = 0; j2 < JMAX; ++j2)
-                    if (j2 < ax[1].J) {
+                    if (FIXED || j2 < ax[1].J) {
                         const cx<T>* row = tile + (l[1] + j2) * T0 + l[0];
                         T ar = 0, ai = 0, br = 0, bi = 0;
 #pragma unroll
                         for (int j1 = 0; j1 < JMAX; ++j1)
-                            if (j1 < ax[0].J) {
+                            if (FIXED || j1 < ax[0].J) {
                                 const cx<T> v = row[j1];
                                 cmadd<T, CK>(ar, ai, cr[0][j1], ci[0][j1], v.x, v.y);
                                 cmadd<T, CK>(br, bi, dr[0][j1], di[0][j1], v.x, v.y);
@@ -476,17 +500,18 @@ __global__ __launch_bounds__(256) void grad_tiled_kernel(TiledArgs<T, CK> a, Gra
                         cmadd<T, CK>(Sr[2], Si[2], dr[1][j2], di[1][j2], ar, ai);
                     }
             } else {
+#pragma unroll(FIXED ? JMAX : 1)
                 for (int j3 = 0; j3 < JMAX; ++j3)
-                    if (j3 < ax[2].J) {
+                    if (FIXED || j3 < ax[2].J) {
                         T pr = 0, pi = 0, qr = 0, qi = 0, ur = 0, ui = 0;
 #pragma unroll
                         for (int j2 = 0; j2 < JMAX; ++j2)
-                            if (j2 < ax[1].J) {
+                            if (FIXED || j2 < ax[1].J) {
                                 const cx<T>* row = tile + ((l[2] + j3) * T1 + (l[1] + j2)) * T0 + l[0];
                                 T ar = 0, ai = 0, br = 0, bi = 0;
 #pragma unroll
                                 for (int j1 = 0; j1 < JMAX; ++j1)
-                                    if (j1 < ax[0].J) {
+                                    if (FIXED || j1 < ax[0].J) {
                                         const cx<T> v = row[j1];
                                         cmadd<T, CK>(ar, ai, cr[0][j1], ci[0][j1], v.x, v.y);
                                         cmadd<T, CK>(br, bi, dr[0][j1], di[0][j1], v.x, v.y);

@@ -60,11 +60,16 @@ __global__ __launch_bounds__(256) void deapod_pad_kernel(const cx<T>* __restrict
 // phi = ((c0*k0) + (c1*k1)) + (c2*k2) with c_d = (T)((2 pi / K_d) * n_mid_d),
 // every product and sum rounded to T; then exp(i phi).
 template <typename T>
-__device__ __forceinline__ cx<T> phase_before_at(const T* c, int ndim, int k1, int k2, int k3) {
+__device__ __forceinline__ T phase_before_arg(const T* c, int ndim, int k1, int k2, int k3) {
     T phi = c[0] * (T)k1;
     if (ndim > 1) phi = phi + c[1] * (T)k2;
     if (ndim > 2) phi = phi + c[2] * (T)k3;
-    return cis<T>(phi);
+    return phi;
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> phase_before_at(const T* c, int ndim, int k1, int k2, int k3) {
+    return cis<T>(phase_before_arg<T>(c, ndim, k1, k2, k3));
 }
 
 // In-place grid *= phase_before (forward, then / scale_ortho: _nufft.py:1371,

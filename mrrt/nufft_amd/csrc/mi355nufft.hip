@@ -432,8 +432,8 @@ int fft_impl(mi355nufft_plan* p, void* grid, int32_t nreps, int direction) {
     return MI355NUFFT_OK;
 }
 
-// Default bin geometry: 2-D 32x32 bins, 3-D 16x16x8 (complex64) or 16x8x8
-// (complex128), 1-D 512; P points per subproblem.  Requested sizes override.
+// Default bin geometry: 2-D 32x32 bins, 3-D 16x8x8, 1-D 512; P points per
+// subproblem.  Requested sizes override.
 void default_bins(const mi355nufft_plan* p, int B[3], int& P) {
     for (int d = 0; d < 3; ++d) B[d] = 1;
     if (p->ndim == 1) {
@@ -445,7 +445,7 @@ void default_bins(const mi355nufft_plan* p, int B[3], int& P) {
         P = 256;
     } else {
         B[0] = 16;
-        B[1] = p->precision == 0 ? 16 : 8;
+        B[1] = 8;
         B[2] = 8;
         P = 512;
     }
@@ -478,7 +478,11 @@ void setup_bin_geom(mi355nufft_plan* p) {
     bg.P = P;
 }
 
-size_t tile_bytes(const mi355nufft_plan* p) { return (size_t)p->bg.tile * p->cx_size(); }
+// LDS of a tiled launch: the gather stages complex T, the scatter
+// accumulates complex float64 (see spread_tiled_kernel); the larger of the two
+// must fit the budget since both share the binning.
+size_t tile_bytes(const mi355nufft_plan* p) { return (size_t)p->bg.tile * 2 * sizeof(double); }
+size_t gather_tile_bytes(const mi355nufft_plan* p) { return (size_t)p->bg.tile * p->cx_size(); }
 
 bool use_tiled(const mi355nufft_plan* p) { return p->method != 1 && p->binned; }
 
@@ -559,12 +563,20 @@ int bin_points_impl(mi355nufft_plan* p) {
 
 int bin_points(mi355nufft_plan* p) { return p->precision ? bin_points_impl<double>(p) : bin_points_impl<float>(p); }
 
-template <typename T, bool CK, int NDIM, int JMAX>
+template <typename T, bool CK, int NDIM, int JMAX, bool FIXED>
 void launch_tiled(bool adjoint, const TiledArgs<T, CK>& a, unsigned nsub, size_t lds, hipStream_t s) {
     if (adjoint)
-        hipLaunchKernelGGL((spread_tiled_kernel<T, NDIM, CK, JMAX>), dim3(nsub), dim3(kBlock), lds, s, a);
+        hipLaunchKernelGGL((spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED>), dim3(nsub), dim3(kBlock), lds, s, a);
     else
-        hipLaunchKernelGGL((interp_tiled_kernel<T, NDIM, CK, JMAX>), dim3(nsub), dim3(kBlock), lds, s, a);
+        hipLaunchKernelGGL((interp_tiled_kernel<T, NDIM, CK, JMAX, FIXED>), dim3(nsub), dim3(kBlock), lds, s, a);
+}
+
+// Taps compiled as fixed trip counts when every axis uses the same J among
+// these; other widths run the generic kernels (J <= JMAX checked per tap).
+int fixed_j(int ndim, const int32_t* Jd) {
+    for (int d = 1; d < ndim; ++d)
+        if (Jd[d] != Jd[0]) return 0;
+    return Jd[0] == 6 ? 6 : 0;
 }
 
 template <typename T, bool CK>
@@ -586,11 +598,20 @@ int run_tiled(mi355nufft_plan* p, bool adjoint, const void* samples_in, void* sa
     a.ortho = adjoint ? 0 : p->ortho;
     a.scale_ortho = (T)p->scale_ortho;
     const unsigned nsub = tiled_grid(p);
-    const size_t lds = tile_bytes(p);
+    const size_t lds = adjoint ? tile_bytes(p) : gather_tile_bytes(p);
     const int jm = jmax_of(p->ndim, J);
+    const int jf = fixed_j(p->ndim, J);
+#define MI355_FCASE(ND, JF)                                                    \
+    if (p->ndim == ND && jf == JF) {                                           \
+        launch_tiled<T, CK, ND, JF, true>(adjoint, a, nsub, lds, p->stream);   \
+        LAUNCH_CHECK();                                                        \
+        return MI355NUFFT_OK;                                                  \
+    }
+    MI355_FCASE(1, 6) MI355_FCASE(2, 6) MI355_FCASE(3, 6)
+#undef MI355_FCASE
 #define MI355_TCASE(ND, JM)                                                  \
     if (p->ndim == ND && jm == JM) {                                         \
-        launch_tiled<T, CK, ND, JM>(adjoint, a, nsub, lds, p->stream);       \
+        launch_tiled<T, CK, ND, JM, false>(adjoint, a, nsub, lds, p->stream); \
         LAUNCH_CHECK();                                                      \
         return MI355NUFFT_OK;                                                \
     }
@@ -608,9 +629,9 @@ int run_tiled_any(mi355nufft_plan* p, bool adjoint, const void* samples_in, void
                       : run_tiled<T, false>(p, adjoint, samples_in, samples_out, grid_in, grid_out, nreps);
 }
 
-template <typename T, bool CK, int NDIM, int JMAX>
+template <typename T, bool CK, int NDIM, int JMAX, bool FIXED>
 void launch_grad(const TiledArgs<T, CK>& a, const GradArgs<T>& g, unsigned nsub, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL((grad_tiled_kernel<T, NDIM, CK, JMAX>), dim3(nsub), dim3(kBlock), lds, s, a, g);
+    hipLaunchKernelGGL((grad_tiled_kernel<T, NDIM, CK, JMAX, FIXED>), dim3(nsub), dim3(kBlock), lds, s, a, g);
 }
 
 template <typename T, bool CK>
@@ -635,11 +656,20 @@ int run_grad(mi355nufft_plan* p, const void* grid, const void* w, void* grad, in
         g.shift[d] = (T)p->shift[d];
     }
     const unsigned nsub = tiled_grid(p);
-    const size_t lds = tile_bytes(p);
+    const size_t lds = gather_tile_bytes(p);
     const int jm = jmax_of(p->ndim, J);
+    const int jf = fixed_j(p->ndim, J);
+#define MI355_GFCASE(ND, JF)                                           \
+    if (p->ndim == ND && jf == JF) {                                   \
+        launch_grad<T, CK, ND, JF, true>(a, g, nsub, lds, p->stream);  \
+        LAUNCH_CHECK();                                                \
+        return MI355NUFFT_OK;                                          \
+    }
+    MI355_GFCASE(1, 6) MI355_GFCASE(2, 6) MI355_GFCASE(3, 6)
+#undef MI355_GFCASE
 #define MI355_GCASE(ND, JM)                                            \
     if (p->ndim == ND && jm == JM) {                                   \
-        launch_grad<T, CK, ND, JM>(a, g, nsub, lds, p->stream);        \
+        launch_grad<T, CK, ND, JM, false>(a, g, nsub, lds, p->stream); \
         LAUNCH_CHECK();                                                \
         return MI355NUFFT_OK;                                          \
     }

@@ -502,9 +502,20 @@ class NufftOracle:
             self._fwd = lambda ck: fwd(ck, self.Kd, self.h, self.Jd, self.Ld, self.tm)
             self._adj = lambda fm: adj(fm, self.Kd, self.h, self.Jd, self.Ld, self.tm)
 
-    def fft(self, x):
-        """``nufft_forward`` (``_nufft.py:1275-1397``), Fortran order."""
+    def fft(self, x, grid_only=False):
+        """``nufft_forward`` (``_nufft.py:1275-1397``), Fortran order.
+
+        ``grid_only``: ``x`` is the oversampled grid (prod(Kd), reps); only the
+        table interpolation (and the complex kernel's phase_shift) is applied
+        and the (M, reps) result is returned unsqueezed (:1302-1304, :1376-1388).
+        """
         Nd, Kd = self.Nd, self.Kd
+        if grid_only:
+            xk = np.asfortranarray(x).reshape((int(np.prod(Kd)), -1), order="F").astype(self.cdt)
+            y = self._fwd(xk)
+            if self.phase_shift is not None:
+                y = (y * self.phase_shift[:, None]).astype(self.cdt)
+            return y.reshape((self.M, xk.shape[1]), order="F")
         x = np.asfortranarray(x).reshape(list(Nd) + [-1], order="F").astype(self.cdt)
         n_reps = x.shape[-1]
         xk = x * self.sn[..., None]
@@ -522,17 +533,27 @@ class NufftOracle:
             y *= self.phase_after[:, None]
         return y[:, 0] if n_reps == 1 else y
 
-    def adj(self, y):
-        """``nufft_adj`` (``_nufft.py:1459-1578``), Fortran order."""
+    def adj(self, y, return_psf=False, grid_only=False):
+        """``nufft_adj`` (``_nufft.py:1459-1578``), Fortran order.
+
+        ``grid_only``: return the raw gridded samples (prod(Kd), reps) before
+        conj(phase_before) and the FFT (:1507-1509); ``return_psf``: skip
+        conj(phase_after) and return the first repetition's grid as Kd
+        (:1495, :1517-1518).
+        """
         Nd, Kd = self.Nd, self.Kd
         y = np.asfortranarray(y).astype(self.cdt).reshape((self.M, -1), order="F").copy()
         n_reps = y.shape[-1]
-        if self.phase_after is not None:
+        if self.phase_after is not None and not return_psf:
             y *= self.phase_after.conj()[:, None]
         if self.phase_shift is not None:          # _nufft.py:1141-1150
             y = (y * self.phase_shift.conj()[:, None]).astype(self.cdt)
         g = self._adj(y)
+        if grid_only:
+            return g
         g = g.reshape(Kd + (n_reps,), order="F")
+        if return_psf:
+            return g[..., 0]
         if self.phase_before is not None:
             g *= self.phase_before.conj()[..., None]
         x = np.fft.ifftn(g, Kd, axes=tuple(range(self.ndim))).astype(self.cdt)

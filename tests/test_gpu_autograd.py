@@ -97,7 +97,10 @@ def test_grads_fp64_vs_finite_differences(phasing, ndim):
 
 
 def test_c64_gradients_vs_fp64_fd_spiral():
-    """Multi-coil spiral, complex64 operator; FP64 FD from a complex128 operator."""
+    """Multi-coil spiral with y from a perturbed trajectory.  The complex128
+    operator's gradients are checked against FP64 central finite
+    differences; the complex64 operator's gradients against those
+    (relative L2), which carries the FD check to complex64."""
     Nd, Kd = (48, 48), (96, 96)
     om = trj.spiral_2d(4, 256, seed=0)
     kw = dict(Jd=6, Kd=Kd, Ld=1024, device=dev())
@@ -108,19 +111,20 @@ def test_c64_gradients_vs_fp64_fd_spiral():
     om_pert = torch.from_numpy(om + 1e-3 * np.random.default_rng(6).standard_normal(om.shape)).to(dev())
     with torch.no_grad():
         y_t = nufft(op64, x64, om_pert)
-    x32 = x64.to(torch.complex64).requires_grad_(True)
-    w32 = om64.to(torch.float32).requires_grad_(True)
-    L32 = ((nufft(op32, x32, w32) - y_t.to(torch.complex64)).abs() ** 2).sum()
-    gx, gw = torch.autograd.grad(L32, (x32, w32))
+    # FP64 gradients vs FP64 finite differences
+    xg = x64.clone().requires_grad_(True)
+    wg = om64.clone().requires_grad_(True)
     loss64 = _loss_fn(op64, y_t)
+    gx64, gw64 = torch.autograd.grad(loss64(xg, wg), (xg, wg))
     v = torch.from_numpy(trj.complex_gaussian((4,) + Nd, seed=7)).to(dev())
     with torch.no_grad():
         fd = ((loss64(x64 + H * v, om64) - loss64(x64 - H * v, om64)) / (2 * H)).item()
-    an = torch.real(torch.sum(torch.conj(gx.to(torch.complex128)) * v)).item()
-    assert abs(fd - an) <= 1e-4 * abs(fd)
+    an = torch.real(torch.sum(torch.conj(gx64) * v)).item()
+    assert abs(fd - an) <= 1e-6 * abs(fd)
     cross = _crossing_mask(op64, om, H)
-    gwn = gw.double().cpu().numpy()
+    gwn = gw64.cpu().numpy()
     scale = np.abs(gwn).max()
+    checked = 0
     for m in np.random.default_rng(8).choice(om.shape[0], 16, replace=False):
         for d in range(2):
             if cross[m, d]:
@@ -129,7 +133,24 @@ def test_c64_gradients_vs_fp64_fd_spiral():
             e[m, d] = H
             with torch.no_grad():
                 fdv = (loss64(x64, om64 + e) - loss64(x64, om64 - e)).item() / (2 * H)
-            assert abs(fdv - gwn[m, d]) <= 2e-3 * scale, (m, d, fdv, gwn[m, d])
+            assert abs(fdv - gwn[m, d]) <= 1e-5 * max(abs(gwn[m, d]), 1e-3 * scale), (m, d, fdv, gwn[m, d])
+            checked += 1
+    assert checked >= 16
+    # complex64 operator vs the FD-checked FP64 gradients.  The complex64
+    # operator differs from the complex128 one by ~1e-5 relative (it keeps the
+    # reference's float32 phase_before, _nufft.py:703-715); the loss residual
+    # r = A x - y is a small fraction of y, so that difference is amplified by
+    # kappa = |y| / |r| in the gradients.
+    x32 = x64.to(torch.complex64).requires_grad_(True)
+    w32 = om64.to(torch.float32).requires_grad_(True)
+    L32 = ((nufft(op32, x32, w32) - y_t.to(torch.complex64)).abs() ** 2).sum()
+    gx32, gw32 = torch.autograd.grad(L32, (x32, w32))
+    with torch.no_grad():
+        y64 = nufft(op64, x64, om64)
+        kappa = (torch.linalg.vector_norm(y64) / torch.linalg.vector_norm(y64 - y_t)).item()
+    assert 1 < kappa < 1e3
+    assert rel_l2(gx32.cpu().numpy(), gx64.cpu().numpy()) <= 5e-5 * kappa
+    assert rel_l2(gw32.double().cpu().numpy(), gwn) <= 2e-4 * kappa
 
 
 def test_adjoint_autograd():

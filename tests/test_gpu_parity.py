@@ -381,3 +381,50 @@ def test_tiled_rejects_tile_over_lds_budget():
     A = NufftBase((40, 36), trj.golden_angle_2d(10, 80), Jd=6, Kd=(80, 75), precision="double", device=dev())
     with pytest.raises(ValueError, match="LDS"):
         A.set_binning((200, 200), 64, "tiled")
+
+
+def test_graph_capture_plain_operator():
+    """forward + adjoint captured in one HIP graph; replays equal the eager
+    result, with eager work and synchronisations between replays."""
+    Nd, Kd = (64, 64), (128, 128)
+    om = trj.golden_angle_2d(40, 128)
+    for precision in ("single", "double"):
+        A = NufftBase(Nd, om, Jd=6, Kd=Kd, Ld=1024, precision=precision, device=dev())
+        cdt = A._tcdt
+        x = torch.from_numpy(trj.complex_gaussian(Nd, seed=30)).to(dev()).to(cdt).unsqueeze(-1)
+        y_ref = A._forward_dev(x, 1).clone()
+        xa_ref = A._adjoint_dev(y_ref, 1).clone()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            A._forward_dev(x, 1)
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            y = A._forward_dev(x, 1)
+            xa = A._adjoint_dev(y, 1)
+        for _ in range(3):
+            g.replay()
+            torch.cuda.synchronize()
+            assert rel_l2(y.cpu().numpy(), y_ref.cpu().numpy()) <= 1e-6
+            assert rel_l2(xa.cpu().numpy(), xa_ref.cpu().numpy()) <= 1e-6
+            _ = (x * 3).abs().sum().item()
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+@pytest.mark.parametrize("phasing", ["real", "complex"])
+def test_grid_only_and_psf_vs_oracle(precision, phasing):
+    """nufft_forward(grid_only) / nufft_adj(grid_only, return_psf)
+    (_nufft.py:1302-1304, :1376-1388, :1495, :1507-1518)."""
+    from mrrt.nufft_amd import nufft_adj, nufft_forward
+    Nd, Kd = (24, 20), (48, 40)
+    om = trj.golden_angle_2d(12, 48)
+    A, B = _pair(Nd, om, Jd=6, Kd=Kd, Ld=1024, precision=precision, phasing=phasing, n_shift=(3.5, 2.0))
+    g = trj.complex_gaussian((int(np.prod(Kd)), 2), seed=40)
+    assert rel_l2(nufft_forward(A, g, grid_only=True), B.fft(g, grid_only=True)) <= TOL[precision]
+    y = trj.complex_gaussian((om.shape[0], 2), seed=41)
+    assert rel_l2(nufft_adj(A, y, grid_only=True), B.adj(y, grid_only=True)) <= TOL[precision]
+    psf = nufft_adj(A, np.ones(om.shape[0]), return_psf=True)
+    assert psf.shape == Kd
+    assert rel_l2(psf, B.adj(np.ones(om.shape[0]), return_psf=True)) <= TOL[precision]
