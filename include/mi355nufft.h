@@ -124,20 +124,34 @@ int64_t mi355nufft_num_points(const mi355nufft_plan *plan);
  * Bin geometry and kernel choice for the interpolation stages (no counterpart
  * in the reference, whose kernels run one thread per sample unsorted).
  *   bin_size    ndim bin sizes in grid points (NULL or 0 entries: defaults
- *               2-D 32x32, 3-D 16x8x8, 1-D 512; the scatter tile is
+ *               2-D 16x16, 3-D 16x8x8, 1-D 512; the scatter tile is
  *               accumulated in complex float64 and must fit the LDS budget)
  *   max_points  points per subproblem (0: default 256 in 2-D, 512 else)
  *   method      0 auto (tiled when the tile fits in LDS), 1 global-memory
- *               kernels on unsorted points, 2 tiled (error if it cannot)
+ *               kernels on unsorted points, 2 tiled (error if it cannot),
+ *               3 tiled with the scatter flushed by global atomics,
+ *               4 tiled with the two-phase scatter (tile slots + one
+ *               reduction per grid point in a fixed order: bit-reproducible
+ *               adjoints; needs the slots to fit 8 GiB and bins not smaller
+ *               than the kernel overlap allows)
  * Re-sorts the current points if any.
  */
 int mi355nufft_set_binning(mi355nufft_plan *plan, const int32_t *bin_size,
                            int32_t max_points, int method);
 
 /*
- * Plan description: info[0..12] = ndim, M, bin size (3), points per
+ * mi355nufft_set_binning plus the workgroup size of the tiled kernels:
+ * threads = 64, 128 or 256 (0: default 256).  No counterpart in the
+ * reference.
+ */
+int mi355nufft_set_tiling(mi355nufft_plan *plan, const int32_t *bin_size,
+                          int32_t max_points, int method, int32_t threads);
+
+/*
+ * Plan description: info[0..14] = ndim, M, bin size (3), points per
  * subproblem, number of bins, tile points, tiled kernels active (0/1),
- * subproblem slots, method, LDS budget in bytes, number of frames.
+ * subproblem slots, method, LDS budget in bytes, number of frames,
+ * two-phase scatter active (0/1), threads per tiled workgroup.
  */
 int mi355nufft_plan_info(const mi355nufft_plan *plan, int64_t *info, int32_t n);
 

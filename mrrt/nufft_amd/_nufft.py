@@ -298,29 +298,34 @@ class NufftBase(object):
             offs, nframes))
         self.nframes = nframes
 
-    _METHODS = {"auto": 0, "gm": 1, "tiled": 2}
+    _METHODS = {"auto": 0, "gm": 1, "tiled": 2, "tiled_atomic": 3, "tiled_two_phase": 4}
 
-    def set_binning(self, bin_size=None, max_points=0, method="auto"):
+    def set_binning(self, bin_size=None, max_points=0, method="auto", threads=0):
         """Bin geometry and kernel family of the interpolation stages.
 
         ``bin_size``: per-axis bin size in grid points (None: defaults);
         ``max_points``: points per subproblem (0: default); ``method``:
         'auto' (tiled when the tile fits in LDS), 'gm' (global-memory kernels)
-        or 'tiled'.  The current points are re-sorted.
+        'tiled', 'tiled_atomic' (scatter flushed with global atomics) or
+        'tiled_two_phase' (scatter through tile slots and a deterministic
+        per-grid-point reduction).  ``threads``: workgroup size of the tiled
+        kernels (64, 128, 256; 0: default).  The current points are re-sorted.
         """
         bs = None
         if bin_size is not None:
             bs = _lib.i32arr(_as_tuple(bin_size, type=int, n=self.ndim))
         self._sync_stream()
-        _lib.check(self._lib.mi355nufft_set_binning(self._plan, bs, int(max_points), self._METHODS[method]))
+        _lib.check(self._lib.mi355nufft_set_tiling(self._plan, bs, int(max_points), self._METHODS[method],
+                                                   int(threads)))
 
     def plan_info(self):
         """dict with the binning state of the plan (see mi355nufft_plan_info)."""
-        buf = (ctypes.c_int64 * 13)()
-        _lib.check(self._lib.mi355nufft_plan_info(self._plan, buf, 13))
+        buf = (ctypes.c_int64 * 15)()
+        _lib.check(self._lib.mi355nufft_plan_info(self._plan, buf, 15))
         v = list(buf)
         return dict(ndim=v[0], M=v[1], bin_size=tuple(v[2:5]), max_points=v[5], nbins=v[6], tile=v[7],
-                    tiled=bool(v[8]), subproblem_slots=v[9], method=v[10], lds_limit=v[11], nframes=v[12])
+                    tiled=bool(v[8]), subproblem_slots=v[9], method=v[10], lds_limit=v[11], nframes=v[12],
+                    two_phase=bool(v[13]), threads=v[14])
 
     def _h_dev(self):
         if getattr(self, "_h_dev_cache", None) is None:

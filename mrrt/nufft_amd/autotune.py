@@ -1,6 +1,7 @@
 """Autotuner for the tiled interpolation kernels.
 
-Picks the bin size and the points per subproblem (workgroup) of a plan by
+Picks the bin size, the points per subproblem and the workgroup size of the
+tiled kernels of a plan by
 timing the gather and the scatter stages on the plan's own points with HIP
 events, and caches the choice per geometry in a JSON file
 (``$MI355NUFFT_TUNE_CACHE`` or ``~/.cache/mi355nufft/autotune.json``).
@@ -21,11 +22,10 @@ from . import _lib
 
 __all__ = ["tune", "candidates", "cache_path"]
 
-_C2 = [((16, 16), 128), ((16, 16), 256), ((32, 32), 128), ((32, 32), 256), ((32, 32), 512),
-       ((32, 32), 1024), ((64, 32), 256), ((64, 64), 512), ((64, 64), 1024)]
-_C3 = [((8, 8, 8), 256), ((8, 8, 8), 512), ((16, 8, 8), 256), ((16, 8, 8), 512), ((16, 16, 8), 512),
-       ((16, 16, 8), 1024), ((16, 16, 4), 512), ((32, 8, 8), 512)]
-_C1 = [((256,), 256), ((512,), 512), ((1024,), 512), ((2048,), 1024)]
+# (bin size, points per subproblem, threads per workgroup)
+_C2 = [((b, b), p, t) for b in (16, 32) for p in (64, 128, 256, 512) for t in (64, 128, 256) if p >= t]
+_C3 = [(b, p, t) for b in ((8, 8, 8), (16, 8, 8)) for p in (256, 512, 1024) for t in (64, 256)]
+_C1 = [((b,), p, t) for b in (256, 512, 1024) for p in (256, 512) for t in (64, 256)]
 
 
 def candidates(ndim):
@@ -90,7 +90,7 @@ def tune(op, nreps=1, cands=None, reps=3, use_cache=True, verbose=False):
     table = _load() if use_cache else {}
     if use_cache and key in table:
         b, p = table[key]["bin_size"], table[key]["max_points"]
-        op.set_binning(tuple(b), p, "auto")
+        op.set_binning(tuple(b), p, "auto", table[key].get("threads", 0))
         return tuple(b), p, table[key].get("timings", {})
     cands = cands or candidates(op.ndim)
     nimg = getattr(op, "nframes", 1) * nreps
@@ -101,9 +101,10 @@ def tune(op, nreps=1, cands=None, reps=3, use_cache=True, verbose=False):
     vp = ctypes.c_void_p
     timings = {}
     best = None
-    for b, p in cands:
+    mode = "tiled_atomic"
+    for b, p, th in cands:
         try:
-            op.set_binning(b, p, "tiled")
+            op.set_binning(b, p, mode, th)
         except ValueError:
             continue
         op._sync_stream()
@@ -111,17 +112,17 @@ def tune(op, nreps=1, cands=None, reps=3, use_cache=True, verbose=False):
                    reps)
         ts = _time(lambda: _lib.check(lib.mi355nufft_spread(op._plan, vp(y.data_ptr()), vp(g.data_ptr()), nreps)),
                    reps)
-        timings[str((b, p))] = (tf, ts)
+        timings[str((b, p, th))] = (tf, ts)
         if verbose:
-            print("autotune", b, p, "gather %.4f ms scatter %.4f ms" % (tf, ts))
+            print("autotune", b, p, th, "gather %.4f ms scatter %.4f ms" % (tf, ts))
         if best is None or tf + ts < best[0]:
-            best = (tf + ts, b, p)
+            best = (tf + ts, b, p, th)
     if best is None:
         op.set_binning(None, 0, "auto")
         return None, 0, timings
-    _, b, p = best
-    op.set_binning(b, p, "auto")
+    _, b, p, th = best
+    op.set_binning(b, p, "auto", th)
     if use_cache:
-        table[key] = {"bin_size": list(b), "max_points": p, "timings": timings}
+        table[key] = {"bin_size": list(b), "max_points": p, "threads": th, "timings": timings}
         _store(table)
     return tuple(b), p, timings

@@ -41,6 +41,7 @@ struct BinState {
     int* sub_bin;    // (max_sub) bin of each subproblem
     int* sub_start;  // (max_sub) first sorted position of each subproblem
     int* n_sub;      // (1) number of subproblems
+    int* bin_sub;    // (nbins) first subproblem of each bin
     const int64_t* frame_off;  // (nframes + 1) first point of each frame, or null
     int nframes;
 };
@@ -110,6 +111,7 @@ __global__ __launch_bounds__(1024) void bin_scan_kernel(BinState bs, int nbins, 
     for (int b = b0; b < b1; ++b) {
         const int c = bs.counts[b];
         bs.bin_start[b] = run_p;
+        bs.bin_sub[b] = run_s;
         const int ns = (c + P - 1) / P;
         for (int i = 0; i < ns && run_s + i < max_sub; ++i) {
             bs.sub_bin[run_s + i] = b;
@@ -141,7 +143,23 @@ struct TiledArgs {
     int use_pb;             // multiply by phase_before (gather) / conj (scatter)
     int ortho;              // gather: divide staged values by scale_ortho
     T scale_ortho;
+    cx<T>* tiles;           // scatter, two-phase: (sub, rep) tile slots; null = atomic flush
+    const T* tms;           // tm in sorted order: tms[d*M + i] = tm[d*M + perm[i]]
+    const cx<T>* phs;       // per-sample phase in sorted order, or null
 };
+
+// Copy tm and the per-sample phase into sorted (bin) order, so the tiled
+// kernels read them coalesced instead of through perm.
+template <typename T>
+__global__ __launch_bounds__(256) void sort_points_kernel(const T* __restrict__ tm, const cx<T>* __restrict__ phase,
+                                                          int64_t M, int ndim, const int* __restrict__ perm,
+                                                          T* __restrict__ tms, cx<T>* __restrict__ phs) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    const int64_t m = perm[i];
+    for (int d = 0; d < ndim; ++d) tms[(int64_t)d * M + i] = tm[(int64_t)d * M + m];
+    if (phase) phs[i] = phase[m];
+}
 
 struct Sub {
     int start, end;
@@ -242,7 +260,7 @@ __global__ __launch_bounds__(256) void interp_tiled_kernel(TiledArgs<T, CK> a) {
             int l[3];
 #pragma unroll
             for (int d = 0; d < NDIM; ++d) {
-                const T t = ia.tm[(int64_t)d * ia.M + m];
+                const T t = a.tms[(int64_t)d * ia.M + i];
                 int k0;
                 if constexpr (CK) k0 = axis_coefs<T, JMAX, FIXED>(t, ia.L, ax[d], ia.h[d], cr[d], ci[d]);
                 else k0 = axis_coefs<T, JMAX, FIXED>(t, ia.L, ax[d], ia.h[d], cr[d]);
@@ -292,8 +310,8 @@ __global__ __launch_bounds__(256) void interp_tiled_kernel(TiledArgs<T, CK> a) {
                     }
             }
             cx<T> o;
-            if (ia.phase) {
-                const cx<T> ph = ia.phase[m];
+            if (a.phs) {
+                const cx<T> ph = a.phs[i];
                 o.x = sr * ph.x - si * ph.y;
                 o.y = sr * ph.y + si * ph.x;
             } else {
@@ -337,15 +355,15 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
             int l[3];
 #pragma unroll
             for (int d = 0; d < NDIM; ++d) {
-                const T t = ia.tm[(int64_t)d * ia.M + m];
+                const T t = a.tms[(int64_t)d * ia.M + i];
                 int k0;
                 if constexpr (CK) k0 = axis_coefs<T, JMAX, FIXED>(t, ia.L, ax[d], ia.h[d], cr[d], ci[d]);
                 else k0 = axis_coefs<T, JMAX, FIXED>(t, ia.L, ax[d], ia.h[d], cr[d]);
                 l[d] = k0 - s.o[d];
             }
             cx<T> f = ia.in[(int64_t)r * ia.M + m];
-            if (ia.phase) {
-                const cx<T> ph = ia.phase[m];
+            if (a.phs) {
+                const cx<T> ph = a.phs[i];
                 const T fr = f.x * ph.x + f.y * ph.y;
                 const T fi = f.y * ph.x - f.x * ph.y;
                 f.x = fr;
@@ -401,6 +419,12 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
             }
         }
         __syncthreads();
+        if (a.tiles) {  // two-phase: write the whole tile to its slot, no phase
+            cx<T>* __restrict__ slot = a.tiles + ((int64_t)sid * ia.nreps + r) * bg.tile;
+            for (int e = tid; e < bg.tile; e += blockDim.x) slot[e] = mkcx<T>((T)tre[e], (T)tim[e]);
+            __syncthreads();
+            continue;
+        }
         cx<T>* __restrict__ g = ia.grid_out + ((int64_t)s.frame * ia.nreps + r) * ia.kstride;
         for (int e = tid; e < bg.tile; e += blockDim.x) {
             const double ar = tre[e], ai = tim[e];
@@ -466,7 +490,7 @@ __global__ __launch_bounds__(256) void grad_tiled_kernel(TiledArgs<T, CK> a, Gra
             int l[3];
 #pragma unroll
             for (int d = 0; d < NDIM; ++d) {
-                const T t = ia.tm[(int64_t)d * ia.M + m];
+                const T t = a.tms[(int64_t)d * ia.M + i];
                 int k0;
                 if constexpr (CK) k0 = axis_coefs_d<T, JMAX, FIXED>(t, ia.L, ax[d], ia.h[d], cr[d], ci[d], dr[d], di[d]);
                 else k0 = axis_coefs_d<T, JMAX, FIXED>(t, ia.L, ax[d], ia.h[d], cr[d], dr[d]);
@@ -527,7 +551,7 @@ __global__ __launch_bounds__(256) void grad_tiled_kernel(TiledArgs<T, CK> a, Gra
                     }
             }
             cx<T> ph = mkcx<T>((T)1, (T)0);
-            if (ia.phase) ph = ia.phase[m];
+            if (a.phs) ph = a.phs[i];
             // y = S0 * ph
             const T yr = Sr[0] * ph.x - Si[0] * ph.y;
             const T yi = Sr[0] * ph.y + Si[0] * ph.x;
@@ -544,6 +568,88 @@ __global__ __launch_bounds__(256) void grad_tiled_kernel(TiledArgs<T, CK> a, Gra
         __syncthreads();
     }
     }
+}
+
+// Second phase of the two-phase scatter: every grid point sums, in a fixed
+// order, the tile slots of the subproblems whose tiles cover it, multiplies by
+// conj(phase_before) and writes the grid (every point is written, so the grid
+// needs no clearing; the result does not depend on scheduling).  The bins
+// covering grid index k along an axis are those whose tile [o, o + T) (mod K)
+// contains k: local offset e = (k - o) mod K (+ K, 2K, ... for tiles wider than
+// the grid) below T.
+template <typename T, int NDIM>
+__global__ __launch_bounds__(256) void tile_reduce_kernel(const cx<T>* __restrict__ tiles, cx<T>* __restrict__ grid,
+                                                          BinGeom bg, BinState bs, int K0, int K1, int K2,
+                                                          int64_t nk, int nreps, T pb0, T pb1, T pb2, int use_pb,
+                                                          int j0, int j1, int j2) {
+    const int jmax[3] = {j0, j1, j2};
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nk) return;
+    const int img = blockIdx.y;            // frame * nreps + rep
+    const int frame = img / nreps;
+    const int r = img - frame * nreps;
+    const int K[3] = {K0, K1, K2};
+    int k[3];
+    {
+        const uint32_t u = (uint32_t)i;
+        k[0] = (int)(u % (uint32_t)K0);
+        const uint32_t q = u / (uint32_t)K0;
+        k[1] = (int)(q % (uint32_t)K1);
+        k[2] = (int)(q / (uint32_t)K1);
+    }
+    // (bin, local offset) pairs covering k along each axis: bins at or below
+    // the one containing k (periodically, jmax of them, set on the host), and
+    // every offset e, e + K, ... inside the tile
+    int cb[3][8], ce[3][8], nc[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        nc[d] = 0;
+        if (d >= NDIM) {
+            cb[d][0] = 0;
+            ce[d][0] = 0;
+            nc[d] = 1;
+            continue;
+        }
+        const int B = bg.B[d], nb = bg.nb[d], Tt = bg.T[d];
+        const int bk = k[d] / B;
+        for (int j = 0; j < jmax[d]; ++j) {
+            int b = bk - j;
+            if (b < 0) b += nb;
+            int e = k[d] - b * B;
+            if (e < 0) e += K[d];
+            for (; e < Tt && nc[d] < 8; e += K[d]) {
+                cb[d][nc[d]] = b;
+                ce[d][nc[d]] = e;
+                ++nc[d];
+            }
+        }
+    }
+    double sr = 0.0, si = 0.0;
+    for (int a2 = 0; a2 < nc[2]; ++a2)
+        for (int a1 = 0; a1 < nc[1]; ++a1)
+            for (int a0 = 0; a0 < nc[0]; ++a0) {
+                const int bin = frame * bg.nbins_frame + (cb[2][a2] * bg.nb[1] + cb[1][a1]) * bg.nb[0] + cb[0][a0];
+                const int cnt = bs.counts[bin];
+                if (cnt == 0) continue;
+                const int nsb = (cnt + bg.P - 1) / bg.P;
+                const int s0 = bs.bin_sub[bin];
+                const int e = (ce[2][a2] * bg.T[1] + ce[1][a1]) * bg.T[0] + ce[0][a0];
+                for (int sub = s0; sub < s0 + nsb; ++sub) {
+                    const cx<T> v = tiles[((int64_t)sub * nreps + r) * bg.tile + e];
+                    sr += (double)v.x;
+                    si += (double)v.y;
+                }
+            }
+    T vr = (T)sr, vi = (T)si;
+    if (use_pb) {  // * conj(phase_before)
+        const T c[3] = {pb0, pb1, pb2};
+        const cx<T> p = cis_tile(phase_before_arg<T>(c, NDIM, k[0], k[1], k[2]));
+        const T wr = vr * p.x + vi * p.y;
+        const T wi = vi * p.x - vr * p.y;
+        vr = wr;
+        vi = wi;
+    }
+    grid[(int64_t)img * nk + i] = mkcx<T>(vr, vi);
 }
 
 }  // namespace mi355

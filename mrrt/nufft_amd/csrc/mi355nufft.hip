@@ -92,14 +92,24 @@ struct mi355nufft_plan {
     size_t grid_bytes = 0;
 
     // binning (tiled kernels)
-    int method = 0;           // 0 auto, 1 global-memory kernels, 2 tiled kernels
+    int method = 0;           // 0 auto, 1 global-memory, 2 tiled, 3 tiled + atomic flush, 4 tiled + two-phase
+    int reduce_ok = 0;        // two-phase scatter possible for the current geometry
+    int reduce_j[3] = {1, 1, 1};  // candidate bins per axis of the reduce kernel
+    void* d_tiles = nullptr;  // two-phase scatter tile slots
+    size_t tiles_bytes = 0;
+    void* tiles_target = nullptr;  // set while a two-phase scatter runs
     int bin_req[3] = {0, 0, 0};  // requested bin size per axis (0: default)
     int P_req = 0;               // requested points per subproblem (0: default)
+    int threads_req = 0;         // requested threads per tiled workgroup (0: default)
     BinGeom bg{};
+    int bg_threads = 256;     // threads per tiled workgroup
     int binned = 0;           // bin state valid for the current points
     int* d_bin_i = nullptr;   // one allocation: bin_of, rank, perm (3*M_cap)
     int* d_bin_b = nullptr;   // counts, bin_start (2*nbins_cap)
     int* d_sub = nullptr;     // sub_bin, sub_start (2*sub_cap) + n_sub
+    void* d_tms = nullptr;    // tm in sorted order (ndim * M_cap real)
+    void* d_phs = nullptr;    // per-sample phase in sorted order (M_cap complex)
+    int64_t sorted_cap = 0;
     int64_t bin_m_cap = 0, bin_b_cap = 0, bin_s_cap = 0;
     size_t lds_limit = 65536;
     int nframes = 1;              // frames with their own points (grids per frame: nreps)
@@ -410,7 +420,12 @@ int interp_impl(mi355nufft_plan* p, const void* grid, void* y, int32_t nreps) {
 }
 
 template <typename T>
+int spread_two_phase(mi355nufft_plan* p, const void* y, void* grid, int32_t nreps);
+bool use_two_phase(const mi355nufft_plan* p, int32_t nreps);
+
+template <typename T>
 int spread_impl(mi355nufft_plan* p, const void* y, void* grid, int32_t nreps) {
+    if (use_tiled(p) && use_two_phase(p, nreps)) return spread_two_phase<T>(p, y, grid, nreps);
     RET_IF(zero_grid<T>(grid, (int64_t)p->nframes * nreps * p->nk, p->stream));
     if (use_tiled(p)) return run_tiled_any<T>(p, true, y, nullptr, nullptr, grid, nreps);
     if (p->nframes > 1) return fail(MI355NUFFT_ESTATE, "frames need the tiled kernels");
@@ -432,16 +447,17 @@ int fft_impl(mi355nufft_plan* p, void* grid, int32_t nreps, int direction) {
     return MI355NUFFT_OK;
 }
 
-// Default bin geometry: 2-D 32x32 bins, 3-D 16x8x8, 1-D 512; P points per
-// subproblem.  Requested sizes override.
+// Default bin geometry (autotuned on golden-angle radial and kooshball
+// trajectories): 2-D 16x16 with P = 256 points per subproblem, 3-D 16x8x8
+// and 1-D 512 with P = 512.  Requested sizes override.
 void default_bins(const mi355nufft_plan* p, int B[3], int& P) {
     for (int d = 0; d < 3; ++d) B[d] = 1;
     if (p->ndim == 1) {
         B[0] = 512;
         P = 512;
     } else if (p->ndim == 2) {
-        B[0] = 32;
-        B[1] = 32;
+        B[0] = 16;
+        B[1] = 16;
         P = 256;
     } else {
         B[0] = 16;
@@ -476,6 +492,24 @@ void setup_bin_geom(mi355nufft_plan* p) {
     bg.nbins_frame = bg.nbins;
     bg.nbins *= p->nframes;
     bg.P = P;
+    p->bg_threads = p->threads_req > 0 ? p->threads_req : 256;
+    // two-phase scatter: every grid point must be covered by at most 8
+    // (bin, offset) pairs per axis, the capacity of tile_reduce_kernel
+    p->reduce_ok = 1;
+    for (int d = 0; d < 3; ++d) {
+        p->reduce_j[d] = 1;
+        if (d >= p->ndim) continue;
+        const int B = bg.B[d], nb = bg.nb[d], T = bg.T[d], K = (int)p->K[d];
+        int jm = (T - 1 + B - 1) / B + 2;
+        if (jm > nb) jm = nb;
+        p->reduce_j[d] = jm;
+        std::vector<int> hits((size_t)K, 0);
+        for (int b = 0; b < nb; ++b)
+            for (int e = 0; e < T; ++e) ++hits[(size_t)(((int64_t)b * B + e) % K)];
+        int mx = 0;
+        for (int v : hits) mx = v > mx ? v : mx;
+        if (mx > 8) p->reduce_ok = 0;
+    }
 }
 
 // LDS of a tiled launch: the gather stages complex T, the scatter
@@ -492,7 +526,8 @@ int64_t max_subproblems(const mi355nufft_plan* p) { return p->bg.nbins + (p->M +
 // 2048 (8 per CU on 256 CUs); workgroups stride over the subproblem list.
 unsigned tiled_grid(const mi355nufft_plan* p) {
     const int64_t n = max_subproblems(p);
-    return (unsigned)(n < 2048 ? n : 2048);
+    const int64_t cap = (int64_t)2048 * 256 / p->bg_threads;
+    return (unsigned)(n < cap ? n : cap);
 }
 
 BinState bin_state(const mi355nufft_plan* p) {
@@ -503,6 +538,7 @@ BinState bin_state(const mi355nufft_plan* p) {
     bs.perm = p->d_bin_i + 2 * mc;
     bs.counts = p->d_bin_b;
     bs.bin_start = p->d_bin_b + bc;
+    bs.bin_sub = p->d_bin_b + 2 * bc;
     bs.sub_bin = p->d_sub;
     bs.sub_start = p->d_sub + sc;
     bs.n_sub = p->d_sub + 2 * sc;
@@ -531,8 +567,10 @@ int bin_points_impl(mi355nufft_plan* p) {
     p->binned = 0;
     if (p->method == 1 || p->M == 0) return MI355NUFFT_OK;
     setup_bin_geom(p);
+    if (p->method == 4 && !p->reduce_ok)
+        return fail(MI355NUFFT_EINVAL, "bins too small for the two-phase scatter");
     if (tile_bytes(p) > p->lds_limit) {
-        if (p->method == 2 || p->nframes > 1)
+        if (p->method >= 2 || p->nframes > 1)
             return fail(MI355NUFFT_EINVAL, "tile of %d points (%zu bytes) exceeds the LDS budget of %zu bytes",
                         p->bg.tile, tile_bytes(p), p->lds_limit);
         return MI355NUFFT_OK;
@@ -540,7 +578,7 @@ int bin_points_impl(mi355nufft_plan* p) {
     const int64_t M = p->M;
     const int64_t max_sub = max_subproblems(p);
     RET_IF(grow(p, &p->d_bin_i, p->bin_m_cap, M, 3));
-    RET_IF(grow(p, &p->d_bin_b, p->bin_b_cap, (int64_t)p->bg.nbins, 2));
+    RET_IF(grow(p, &p->d_bin_b, p->bin_b_cap, (int64_t)p->bg.nbins, 3));
     RET_IF(grow(p, &p->d_sub, p->bin_s_cap, max_sub, 2));
     BinState bs = bin_state(p);
     InterpArgs<T, false> ia;
@@ -557,6 +595,23 @@ int bin_points_impl(mi355nufft_plan* p) {
     LAUNCH_CHECK();
     hipLaunchKernelGGL(bin_scatter_kernel, dim3(nblocks(M)), dim3(kBlock), 0, p->stream, M, bs);
     LAUNCH_CHECK();
+    if (M > p->sorted_cap) {
+        if (p->d_tms) {
+            HIP_TRY(hipStreamSynchronize(p->stream));
+            (void)hipFree(p->d_tms);
+            (void)hipFree(p->d_phs);
+            p->d_tms = p->d_phs = nullptr;
+            p->sorted_cap = 0;
+        }
+        if (hipMalloc(&p->d_tms, (size_t)M * p->ndim * p->real_size()) != hipSuccess ||
+            hipMalloc(&p->d_phs, (size_t)M * p->cx_size()) != hipSuccess)
+            return fail(MI355NUFFT_ENOMEM, "sorted points storage for M=%lld", (long long)M);
+        p->sorted_cap = M;
+    }
+    hipLaunchKernelGGL((sort_points_kernel<T>), dim3(nblocks(M)), dim3(kBlock), 0, p->stream, (const T*)p->d_tm,
+                       p->has_phase ? (const cx<T>*)p->d_phase : nullptr, M, p->ndim, (const int*)bs.perm,
+                       (T*)p->d_tms, (cx<T>*)p->d_phs);
+    LAUNCH_CHECK();
     p->binned = 1;
     return MI355NUFFT_OK;
 }
@@ -564,11 +619,12 @@ int bin_points_impl(mi355nufft_plan* p) {
 int bin_points(mi355nufft_plan* p) { return p->precision ? bin_points_impl<double>(p) : bin_points_impl<float>(p); }
 
 template <typename T, bool CK, int NDIM, int JMAX, bool FIXED>
-void launch_tiled(bool adjoint, const TiledArgs<T, CK>& a, unsigned nsub, size_t lds, hipStream_t s) {
+void launch_tiled(bool adjoint, const TiledArgs<T, CK>& a, unsigned nsub, unsigned threads, size_t lds,
+                  hipStream_t s) {
     if (adjoint)
-        hipLaunchKernelGGL((spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED>), dim3(nsub), dim3(kBlock), lds, s, a);
+        hipLaunchKernelGGL((spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED>), dim3(nsub), dim3(threads), lds, s, a);
     else
-        hipLaunchKernelGGL((interp_tiled_kernel<T, NDIM, CK, JMAX, FIXED>), dim3(nsub), dim3(kBlock), lds, s, a);
+        hipLaunchKernelGGL((interp_tiled_kernel<T, NDIM, CK, JMAX, FIXED>), dim3(nsub), dim3(threads), lds, s, a);
 }
 
 // Taps compiled as fixed trip counts when every axis uses the same J among
@@ -597,13 +653,16 @@ int run_tiled(mi355nufft_plan* p, bool adjoint, const void* samples_in, void* sa
     for (int d = 0; d < 3; ++d) a.pb[d] = (T)p->phase_coef[d];
     a.ortho = adjoint ? 0 : p->ortho;
     a.scale_ortho = (T)p->scale_ortho;
+    a.tiles = adjoint ? (cx<T>*)p->tiles_target : nullptr;
+    a.tms = (const T*)p->d_tms;
+    a.phs = p->has_phase ? (const cx<T>*)p->d_phs : nullptr;
     const unsigned nsub = tiled_grid(p);
     const size_t lds = adjoint ? tile_bytes(p) : gather_tile_bytes(p);
     const int jm = jmax_of(p->ndim, J);
     const int jf = fixed_j(p->ndim, J);
 #define MI355_FCASE(ND, JF)                                                    \
     if (p->ndim == ND && jf == JF) {                                           \
-        launch_tiled<T, CK, ND, JF, true>(adjoint, a, nsub, lds, p->stream);   \
+        launch_tiled<T, CK, ND, JF, true>(adjoint, a, nsub, p->bg_threads, lds, p->stream);   \
         LAUNCH_CHECK();                                                        \
         return MI355NUFFT_OK;                                                  \
     }
@@ -611,7 +670,7 @@ int run_tiled(mi355nufft_plan* p, bool adjoint, const void* samples_in, void* sa
 #undef MI355_FCASE
 #define MI355_TCASE(ND, JM)                                                  \
     if (p->ndim == ND && jm == JM) {                                         \
-        launch_tiled<T, CK, ND, JM, false>(adjoint, a, nsub, lds, p->stream); \
+        launch_tiled<T, CK, ND, JM, false>(adjoint, a, nsub, p->bg_threads, lds, p->stream); \
         LAUNCH_CHECK();                                                      \
         return MI355NUFFT_OK;                                                \
     }
@@ -630,8 +689,9 @@ int run_tiled_any(mi355nufft_plan* p, bool adjoint, const void* samples_in, void
 }
 
 template <typename T, bool CK, int NDIM, int JMAX, bool FIXED>
-void launch_grad(const TiledArgs<T, CK>& a, const GradArgs<T>& g, unsigned nsub, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL((grad_tiled_kernel<T, NDIM, CK, JMAX, FIXED>), dim3(nsub), dim3(kBlock), lds, s, a, g);
+void launch_grad(const TiledArgs<T, CK>& a, const GradArgs<T>& g, unsigned nsub, unsigned threads, size_t lds,
+                 hipStream_t s) {
+    hipLaunchKernelGGL((grad_tiled_kernel<T, NDIM, CK, JMAX, FIXED>), dim3(nsub), dim3(threads), lds, s, a, g);
 }
 
 template <typename T, bool CK>
@@ -648,6 +708,8 @@ int run_grad(mi355nufft_plan* p, const void* grid, const void* w, void* grad, in
     for (int d = 0; d < 3; ++d) a.pb[d] = (T)p->phase_coef[d];
     a.ortho = p->ortho;
     a.scale_ortho = (T)p->scale_ortho;
+    a.tms = (const T*)p->d_tms;
+    a.phs = p->has_phase ? (const cx<T>*)p->d_phs : nullptr;
     GradArgs<T> g;
     g.w = (const cx<T>*)w;
     g.grad = (T*)grad;
@@ -661,7 +723,7 @@ int run_grad(mi355nufft_plan* p, const void* grid, const void* w, void* grad, in
     const int jf = fixed_j(p->ndim, J);
 #define MI355_GFCASE(ND, JF)                                           \
     if (p->ndim == ND && jf == JF) {                                   \
-        launch_grad<T, CK, ND, JF, true>(a, g, nsub, lds, p->stream);  \
+        launch_grad<T, CK, ND, JF, true>(a, g, nsub, p->bg_threads, lds, p->stream);  \
         LAUNCH_CHECK();                                                \
         return MI355NUFFT_OK;                                          \
     }
@@ -669,7 +731,7 @@ int run_grad(mi355nufft_plan* p, const void* grid, const void* w, void* grad, in
 #undef MI355_GFCASE
 #define MI355_GCASE(ND, JM)                                            \
     if (p->ndim == ND && jm == JM) {                                   \
-        launch_grad<T, CK, ND, JM, false>(a, g, nsub, lds, p->stream); \
+        launch_grad<T, CK, ND, JM, false>(a, g, nsub, p->bg_threads, lds, p->stream); \
         LAUNCH_CHECK();                                                \
         return MI355NUFFT_OK;                                          \
     }
@@ -678,6 +740,55 @@ int run_grad(mi355nufft_plan* p, const void* grid, const void* w, void* grad, in
     MI355_GCASE(3, 4) MI355_GCASE(3, 8) MI355_GCASE(3, 16)
 #undef MI355_GCASE
     return fail(MI355NUFFT_EINVAL, "unsupported ndim/J combination");
+}
+
+// Two-phase scatter (binning method 4, the deterministic mode): the tiled
+// scatter writes each subproblem's tile to a slot, tile_reduce_kernel sums
+// the slots covering each grid point in a fixed order and writes the grid
+// once.  Measured slower than the atomic flush on MI355X (slot traffic), so
+// the atomic flush is the default.
+constexpr size_t kTileSlotBudget = (size_t)8 << 30;
+
+size_t two_phase_bytes(const mi355nufft_plan* p, int32_t nreps) {
+    return (size_t)max_subproblems(p) * (size_t)nreps * (size_t)p->bg.tile * p->cx_size();
+}
+
+bool use_two_phase(const mi355nufft_plan* p, int32_t nreps) {
+    return p->method == 4 && p->reduce_ok && two_phase_bytes(p, nreps) <= kTileSlotBudget;
+}
+
+template <typename T>
+int spread_two_phase(mi355nufft_plan* p, const void* y, void* grid, int32_t nreps) {
+    const size_t need = two_phase_bytes(p, nreps);
+    if (need > p->tiles_bytes) {
+        if (p->d_tiles) {
+            HIP_TRY(hipStreamSynchronize(p->stream));
+            (void)hipFree(p->d_tiles);
+            p->d_tiles = nullptr;
+            p->tiles_bytes = 0;
+        }
+        if (hipMalloc(&p->d_tiles, need) != hipSuccess)
+            return fail(MI355NUFFT_ENOMEM, "scatter tile slots of %zu bytes", need);
+        p->tiles_bytes = need;
+    }
+    p->tiles_target = p->d_tiles;
+    const int rc = run_tiled_any<T>(p, true, y, nullptr, nullptr, grid, nreps);
+    p->tiles_target = nullptr;
+    RET_IF(rc);
+    const int nimg = p->nframes * nreps;
+    const T c0 = (T)p->phase_coef[0], c1 = (T)p->phase_coef[1], c2 = (T)p->phase_coef[2];
+    const int use_pb = p->phasing == 0;
+#define MI355_RED(ND)                                                                                          \
+    hipLaunchKernelGGL((tile_reduce_kernel<T, ND>), dim3(nblocks(p->nk), nimg), dim3(kBlock), 0, p->stream,     \
+                       (const cx<T>*)p->d_tiles, (cx<T>*)grid, p->bg, bin_state(p), (int)p->K[0], (int)p->K[1],  \
+                       (int)p->K[2], p->nk, nreps, c0, c1, c2, use_pb, p->reduce_j[0], p->reduce_j[1],          \
+                       p->reduce_j[2])
+    if (p->ndim == 1) MI355_RED(1);
+    else if (p->ndim == 2) MI355_RED(2);
+    else MI355_RED(3);
+#undef MI355_RED
+    LAUNCH_CHECK();
+    return MI355NUFFT_OK;
 }
 
 int need_points(const mi355nufft_plan* p) {
@@ -771,6 +882,9 @@ int mi355nufft_plan_destroy(mi355nufft_plan* p) {
     if (p->d_bin_b) (void)hipFree(p->d_bin_b);
     if (p->d_sub) (void)hipFree(p->d_sub);
     if (p->d_frame_off) (void)hipFree(p->d_frame_off);
+    if (p->d_tiles) (void)hipFree(p->d_tiles);
+    if (p->d_tms) (void)hipFree(p->d_tms);
+    if (p->d_phs) (void)hipFree(p->d_phs);
     delete p;
     return MI355NUFFT_OK;
 }
@@ -866,9 +980,17 @@ int mi355nufft_set_points_frames(mi355nufft_plan* p, const void* omega, int omeg
 }
 
 int mi355nufft_set_binning(mi355nufft_plan* p, const int32_t* bin_size, int32_t max_points, int method) {
+    return mi355nufft_set_tiling(p, bin_size, max_points, method, 0);
+}
+
+int mi355nufft_set_tiling(mi355nufft_plan* p, const int32_t* bin_size, int32_t max_points, int method,
+                          int32_t threads) {
     g_err.clear();
     RET_IF(check_plan(p));
-    if (method < 0 || method > 2) return fail(MI355NUFFT_EINVAL, "method must be 0, 1 or 2");
+    if (threads != 0 && threads != 64 && threads != 128 && threads != 256)
+        return fail(MI355NUFFT_EINVAL, "threads must be 0, 64, 128 or 256");
+    p->threads_req = threads;
+    if (method < 0 || method > 4) return fail(MI355NUFFT_EINVAL, "method must be 0..4");
     if (max_points < 0) return fail(MI355NUFFT_EINVAL, "max_points must be >= 0");
     for (int d = 0; d < p->ndim; ++d) {
         const int b = bin_size ? bin_size[d] : 0;
@@ -878,10 +1000,12 @@ int mi355nufft_set_binning(mi355nufft_plan* p, const int32_t* bin_size, int32_t 
     p->P_req = max_points;
     p->method = method;
     if (p->d_tm && p->M > 0) return bin_points(p);
-    if (method == 2) {
+    if (method >= 2) {
         setup_bin_geom(p);
         if (tile_bytes(p) > p->lds_limit)
             return fail(MI355NUFFT_EINVAL, "tile of %d points exceeds the LDS budget", p->bg.tile);
+        if (method == 4 && !p->reduce_ok)
+            return fail(MI355NUFFT_EINVAL, "bins too small for the two-phase scatter");
     }
     return MI355NUFFT_OK;
 }
@@ -890,10 +1014,10 @@ int mi355nufft_plan_info(const mi355nufft_plan* p, int64_t* info, int32_t n) {
     g_err.clear();
     RET_IF(check_plan(p));
     if (!info || n < 1) return fail(MI355NUFFT_EINVAL, "info buffer required");
-    const int64_t v[13] = {p->ndim, p->M, p->bg.B[0], p->bg.B[1], p->bg.B[2], p->bg.P, p->bg.nbins, p->bg.tile,
+    const int64_t v[15] = {p->ndim, p->M, p->bg.B[0], p->bg.B[1], p->bg.B[2], p->bg.P, p->bg.nbins, p->bg.tile,
                            use_tiled(p) ? 1 : 0, p->binned ? max_subproblems(p) : 0, p->method, (int64_t)p->lds_limit,
-                           p->nframes};
-    for (int i = 0; i < n && i < 13; ++i) info[i] = v[i];
+                           p->nframes, (use_tiled(p) && use_two_phase(p, 1)) ? 1 : 0, p->bg_threads};
+    for (int i = 0; i < n && i < 15; ++i) info[i] = v[i];
     return MI355NUFFT_OK;
 }
 

@@ -330,6 +330,8 @@ BINNINGS = [
     # (method, bin_size, max_points)
     ("gm", None, 0),
     ("tiled", None, 0),
+    ("tiled_atomic", None, 0),
+    ("tiled_two_phase", (8, 8), 100),
     ("tiled", (8, 8), 7),        # many small subproblems
     ("tiled", (64, 16), 1000),   # anisotropic bins
     ("tiled", (200, 40), 64),    # bins larger than the grid on axis 0 (clamped to K)
@@ -346,7 +348,11 @@ def test_binning_variants_vs_oracle(method, bin_size, max_points, precision, pha
                  n_shift=(7.5, -3.0))
     A.set_binning(bin_size, max_points, method)
     info = A.plan_info()
-    assert info["tiled"] == (method == "tiled")
+    assert info["tiled"] == method.startswith("tiled")
+    if method == "tiled_two_phase":
+        assert info["two_phase"]
+    if method == "tiled_atomic":
+        assert not info["two_phase"]
     x = trj.complex_gaussian(Nd + (2,), seed=21)
     y = A.fft(x)
     assert rel_l2(y, B.fft(x)) <= TOL[precision]
@@ -372,8 +378,8 @@ def test_tiled_1d_3d_small_grids(ndim):
 def test_default_binning_active():
     A = NufftBase((64, 64), trj.radial_2d(16, 128), Jd=6, Kd=(128, 128), device=dev())
     info = A.plan_info()
-    assert info["tiled"] and info["bin_size"][:2] == (32, 32)
-    assert info["nbins"] == 16 and info["tile"] == 37 * 37
+    assert info["tiled"] and info["bin_size"][:2] == (16, 16)
+    assert info["nbins"] == 64 and info["tile"] == 21 * 21
 
 
 def test_tiled_rejects_tile_over_lds_budget():
@@ -428,3 +434,29 @@ def test_grid_only_and_psf_vs_oracle(precision, phasing):
     psf = nufft_adj(A, np.ones(om.shape[0]), return_psf=True)
     assert psf.shape == Kd
     assert rel_l2(psf, B.adj(np.ones(om.shape[0]), return_psf=True)) <= TOL[precision]
+
+
+def test_two_phase_scatter_is_deterministic():
+    """The two-phase scatter sums in a fixed order: repeated adjoints are
+    bit-identical (the atomic flush is not, in general)."""
+    Nd, Kd = (64, 64), (128, 128)
+    om = trj.golden_angle_2d(60, 128)
+    A = NufftBase(Nd, om, Jd=6, Kd=Kd, Ld=1024, device=dev())
+    A.set_binning(None, 0, "tiled_two_phase")
+    y = trj.complex_gaussian((om.shape[0],), seed=50).astype(np.complex64)
+    ref = A.adj(y)
+    for _ in range(3):
+        assert np.array_equal(A.adj(y), ref)
+
+
+@pytest.mark.parametrize("ndim", [1, 3])
+def test_two_phase_small_grids_wrap(ndim):
+    """Tiles wider than the grid cover a grid point at several offsets."""
+    if ndim == 1:
+        Nd, Kd, om = (10,), (20,), np.random.default_rng(1).uniform(-np.pi, np.pi, (300, 1))
+    else:
+        Nd, Kd, om = (6, 8, 5), (12, 16, 10), trj.kooshball_3d(40, 16, seed=4)
+    A, B = _pair(Nd, om, Jd=6, Kd=Kd, Ld=512, precision="double")
+    A.set_binning(None, 7, "tiled_two_phase")
+    y = trj.complex_gaussian((om.shape[0],), seed=51)
+    assert rel_l2(A.adj(y), B.adj(y)) <= 1e-12
