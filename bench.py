@@ -28,6 +28,15 @@ sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
 
+KERNEL_NAMES = {"interp": "interp_tiled_kernel<float, 2, false, 6, true>",
+                "spread": "spread_tiled_kernel<float, 2, false, 6, true>"}
+
+# HBM bytes per launch of the 2-D 256^2 complex64 kernels, from rocprofv3
+# FETCH_SIZE and WRITE_SIZE passes over this bench (--calibrate-copy), each
+# scaled by what the counter reads for a 256 MiB device copy (FETCH_SIZE
+# x 2.0, WRITE_SIZE x 1.0); profiles/nufft2d_256_c64_hbm_traffic.md.
+TRAFFIC_BYTES = {}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -38,6 +47,9 @@ def parse():
     ap.add_argument("--no-extra", action="store_true", help="skip the complex128 / 3-D extra measurements")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--graph", action="store_true", help="time a HIP-graph replay instead of eager launches")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run three 256 MiB device-to-device copies (a known-size reference for "
+                         "FETCH_SIZE / WRITE_SIZE counter passes)")
     return ap.parse_args()
 
 
@@ -287,6 +299,15 @@ def main():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         sys.exit(1)
 
+    if args.calibrate_copy:
+        a = torch.empty(64 << 20, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+        print("bench: calibration copies done (3 x 256 MiB)", file=sys.stderr, flush=True)
+
     wl = c2_workload("single")
     run = GpuRun(wl)
     print("bench: plan ready (M=%d); timing %d steps after %d warmup" % (run.M, args.steps, args.warmup),
@@ -298,12 +319,24 @@ def main():
     stages = run.stage_times(max(10, args.steps // 2), 3)
     print("bench: stages (ms) %s" % json.dumps({k: round(v, 5) for k, v in stages.items()}),
           file=sys.stderr, flush=True)
+    # Kernel durations: a second eager run of the same steps with the plan's
+    # event pairs around every gather / scatter launch (on the plan's stream).
+    run.A.set_kernel_timing(True)
+    _time_fn(run.pair, args.steps, 0)
+    run.A.set_kernel_timing(False)
+    kern = {w: run.A.read_kernel_timing(w) for w in ("interp", "spread")}
+    print("bench: kernel means (ms) %s" % json.dumps({k: [v[0], round(v[1], 5)] for k, v in kern.items()}),
+          file=sys.stderr, flush=True)
     b = interp_bytes(run.M, 2, run.nk, 1, 4)
-    dom = "interp" if stages["interp"] >= stages["spread"] else "spread"
-    achieved = b / (stages[dom] * 1e-3) / 1e9
+    dom = "interp" if kern["interp"][1] >= kern["spread"][1] else "spread"
+    achieved = b / (kern[dom][1] * 1e-3) / 1e9
+    traffic = TRAFFIC_BYTES.get(dom)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
-                "bytes_per_launch": b}
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "kernel": KERNEL_NAMES[dom], "kernel_ms": round(kern[dom][1], 5), "launches": kern[dom][0],
+                "bytes_per_launch": b,
+                "other_kernel": {"kernel": KERNEL_NAMES["interp" if dom == "spread" else "spread"],
+                                 "kernel_ms": round(kern["interp" if dom == "spread" else "spread"][1], 5)}}
     extra = {}
     if not args.no_extra:
         r2 = GpuRun(c2_workload("double"))

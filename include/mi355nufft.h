@@ -148,6 +148,18 @@ int mi355nufft_set_tiling(mi355nufft_plan *plan, const int32_t *bin_size,
                           int32_t max_points, int method, int32_t threads);
 
 /*
+ * Kernel timing.  While enabled, every tiled gather (which = 0) or scatter
+ * (which = 1) launch on a stream that is not being captured is bracketed by
+ * a HIP event pair on the plan's stream.  mi355nufft_read_timing waits for
+ * the recorded launches, returns their number and summed duration in
+ * milliseconds, and resets both.  Instrumentation for benchmarks; no
+ * counterpart in the reference.
+ */
+int mi355nufft_set_timing(mi355nufft_plan *plan, int32_t enable);
+int mi355nufft_read_timing(mi355nufft_plan *plan, int32_t which, int64_t *launches,
+                           double *total_ms);
+
+/*
  * Plan description: info[0..14] = ndim, M, bin size (3), points per
  * subproblem, number of bins, tile points, tiled kernels active (0/1),
  * subproblem slots, method, LDS budget in bytes, number of frames,

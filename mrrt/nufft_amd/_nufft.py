@@ -327,6 +327,20 @@ class NufftBase(object):
                     tiled=bool(v[8]), subproblem_slots=v[9], method=v[10], lds_limit=v[11], nframes=v[12],
                     two_phase=bool(v[13]), threads=v[14])
 
+    def set_kernel_timing(self, enable=True):
+        """Bracket every tiled gather / scatter launch with HIP events on the
+        plan's stream (skipped while the stream is being captured)."""
+        _lib.check(self._lib.mi355nufft_set_timing(self._plan, 1 if enable else 0))
+
+    def read_kernel_timing(self, which):
+        """(launches, mean_ms) of the gather ("interp") or scatter ("spread")
+        launches timed since the last read; waits for them to finish."""
+        w = {"interp": 0, "spread": 1}[which]
+        n = ctypes.c_int64(0)
+        ms = ctypes.c_double(0.0)
+        _lib.check(self._lib.mi355nufft_read_timing(self._plan, w, ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, (ms.value / n.value if n.value else float("nan"))
+
     def _h_dev(self):
         if getattr(self, "_h_dev_cache", None) is None:
             self._h_dev_cache = [torch.from_numpy(t).to(self.device) for t in self._tables_host]

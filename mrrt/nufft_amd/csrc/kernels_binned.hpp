@@ -33,8 +33,8 @@ struct BinGeom {
 };
 
 struct BinState {
-    int* bin_of;     // (M) bin of each point
-    int* rank;       // (M) rank inside its bin
+    int* bin_of;     // (M) bin * kBanks + bank key of each point
+    int* rank;       // (M) rank among the points of its (bin, bank key)
     int* counts;     // (nbins) points per bin
     int* bin_start;  // (nbins) first sorted position of each bin
     int* perm;       // (M) sorted position -> point index
@@ -42,9 +42,19 @@ struct BinState {
     int* sub_start;  // (max_sub) first sorted position of each subproblem
     int* n_sub;      // (1) number of subproblems
     int* bin_sub;    // (nbins) first subproblem of each bin
+    int* bkt;        // (nbins * kBanks) points per (bin, bank key)
     const int64_t* frame_off;  // (nframes + 1) first point of each frame, or null
     int nframes;
 };
+
+// Points inside a bin are ordered so that consecutive points start their
+// footprints on different LDS banks.  The bank key of a point is the tile
+// offset of its first tap modulo kBanks (in 8-byte tile entries); the order
+// is by (rank inside the key, key), so any kBanks consecutive points of one
+// rank level have distinct keys and points in the same grid cell are at
+// least one level apart.  The same order serves the gather (tile reads) and
+// the scatter (tile atomics).
+constexpr int kBanks = 32;
 
 template <typename T>
 __device__ __forceinline__ int first_tap_mod(T t, int J, int K) {
@@ -58,11 +68,14 @@ __global__ __launch_bounds__(256) void bin_count_kernel(const T* __restrict__ tm
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= M) return;
     const AxisGeom ax[3] = {a0, a1, a2};
-    int bin = 0, mul = 1;
+    int bin = 0, mul = 1, base = 0, tmul = 1;
     for (int d = 0; d < ndim; ++d) {
         const int k0 = first_tap_mod(tm[(int64_t)d * M + m], ax[d].J, ax[d].K);
-        bin += (k0 / bg.B[d]) * mul;
+        const int b = k0 / bg.B[d];
+        bin += b * mul;
         mul *= bg.nb[d];
+        base += (k0 - b * bg.B[d]) * tmul;
+        tmul *= bg.T[d];
     }
     if (bs.frame_off) {  // frame of point m: last f with frame_off[f] <= m
         int lo = 0, hi = bs.nframes - 1;
@@ -73,8 +86,23 @@ __global__ __launch_bounds__(256) void bin_count_kernel(const T* __restrict__ tm
         }
         bin += lo * bg.nbins_frame;
     }
-    bs.bin_of[m] = bin;
-    bs.rank[m] = atomicAdd(&bs.counts[bin], 1);
+    const int slot = bin * kBanks + (base & (kBanks - 1));
+    bs.bin_of[m] = slot;
+    bs.rank[m] = atomicAdd(&bs.bkt[slot], 1);
+}
+
+// counts[b] = points of bin b (sum over its bank keys)
+__global__ __launch_bounds__(256) void bin_sum_kernel(BinState bs, int nbins) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbins) return;
+    const int4* q = reinterpret_cast<const int4*>(bs.bkt + (int64_t)b * kBanks);
+    int c = 0;
+#pragma unroll
+    for (int k = 0; k < kBanks / 4; ++k) {
+        const int4 v = q[k];
+        c += v.x + v.y + v.z + v.w;
+    }
+    bs.counts[b] = c;
 }
 
 // One workgroup of 1024 threads: exclusive scans of the bin counts and of the
@@ -113,9 +141,10 @@ __global__ __launch_bounds__(1024) void bin_scan_kernel(BinState bs, int nbins, 
         bs.bin_start[b] = run_p;
         bs.bin_sub[b] = run_s;
         const int ns = (c + P - 1) / P;
+        const int q = ns ? c / ns : 0, rem = ns ? c - q * ns : 0;
         for (int i = 0; i < ns && run_s + i < max_sub; ++i) {
             bs.sub_bin[run_s + i] = b;
-            bs.sub_start[run_s + i] = run_p + i * P;
+            bs.sub_start[run_s + i] = run_p + i * q + min(i, rem);
         }
         run_p += c;
         run_s += ns;
@@ -123,10 +152,36 @@ __global__ __launch_bounds__(1024) void bin_scan_kernel(BinState bs, int nbins, 
     if (tid == 1023) bs.n_sub[0] = min(s_sub[1023], max_sub);
 }
 
-__global__ __launch_bounds__(256) void bin_scatter_kernel(int64_t M, BinState bs) {
+// Sorted position of point m.  Its place in the bin's (rank, key) order is
+// pos = #{(k', r') : r' < r} + #{k' < key : count(k') > r}.  A bin of c points
+// has ns = ceil(c / P) subproblems, dealt round-robin over that order:
+// subproblem j holds positions j, j + ns, j + 2 ns, ... (q or q + 1 points,
+// c = q ns + rem), so every subproblem samples all rank levels and the dense
+// cells of a bin do not pile up in its last subproblem.
+__global__ __launch_bounds__(256) void bin_scatter_kernel(int64_t M, BinState bs, int P) {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= M) return;
-    bs.perm[bs.bin_start[bs.bin_of[m]] + bs.rank[m]] = (int)m;
+    const int slot = bs.bin_of[m];
+    const int bin = slot / kBanks;
+    const int key = slot - bin * kBanks;
+    const int r = bs.rank[m];
+    const int4* row = reinterpret_cast<const int4*>(bs.bkt + (int64_t)bin * kBanks);
+    int pos = 0;
+#pragma unroll
+    for (int k4 = 0; k4 < kBanks / 4; ++k4) {
+        const int4 v = row[k4];
+        const int c[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = 4 * k4 + u;
+            pos += min(c[u], r) + ((k < key && c[u] > r) ? 1 : 0);
+        }
+    }
+    const int c = bs.counts[bin];
+    const int ns = (c + P - 1) / P;
+    const int q = c / ns, rem = c - q * ns;
+    const int j = pos % ns;
+    bs.perm[bs.bin_start[bin] + j * q + min(j, rem) + pos / ns] = (int)m;
 }
 
 __global__ __launch_bounds__(256) void fill_int_kernel(int* __restrict__ p, int64_t n, int v) {
@@ -172,7 +227,13 @@ struct Sub {
 __device__ __forceinline__ void load_sub(const BinGeom& bg, const BinState& bs, int ndim, int sid, Sub& s) {
     const int bin = bs.sub_bin[sid];
     s.start = bs.sub_start[sid];
-    s.end = min(s.start + bg.P, bs.bin_start[bin] + bs.counts[bin]);
+    {  // subproblem j of its bin holds q + (j < rem) points (bin_scan_kernel)
+        const int c = bs.counts[bin];
+        const int ns = (c + bg.P - 1) / bg.P;
+        const int q = c / ns, rem = c - q * ns;
+        const int j = sid - bs.bin_sub[bin];
+        s.end = s.start + q + (j < rem ? 1 : 0);
+    }
     s.frame = bin / bg.nbins_frame;
     int q = bin - s.frame * bg.nbins_frame;
     for (int d = 0; d < 3; ++d) {

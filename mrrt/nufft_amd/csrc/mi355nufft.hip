@@ -105,7 +105,7 @@ struct mi355nufft_plan {
     int bg_threads = 256;     // threads per tiled workgroup
     int binned = 0;           // bin state valid for the current points
     int* d_bin_i = nullptr;   // one allocation: bin_of, rank, perm (3*M_cap)
-    int* d_bin_b = nullptr;   // counts, bin_start (2*nbins_cap)
+    int* d_bin_b = nullptr;   // counts, bin_start, bin_sub, bank-key counts ((3 + kBanks) * nbins_cap)
     int* d_sub = nullptr;     // sub_bin, sub_start (2*sub_cap) + n_sub
     void* d_tms = nullptr;    // tm in sorted order (ndim * M_cap real)
     void* d_phs = nullptr;    // per-sample phase in sorted order (M_cap complex)
@@ -115,6 +115,13 @@ struct mi355nufft_plan {
     int nframes = 1;              // frames with their own points (grids per frame: nreps)
     int64_t* d_frame_off = nullptr;
     int64_t frame_cap = 0;
+
+    // launch timing of the tiled gather (0) and scatter (1) kernels
+    int timing = 0;
+    std::vector<hipEvent_t> tev[2];  // start/stop pairs, reused
+    int tev_used[2] = {0, 0};
+    double tev_ms[2] = {0, 0};       // folded totals of earlier pairs
+    int64_t tev_n[2] = {0, 0};
 
     std::vector<FftPlans> ffts;
     rocfft_execution_info fft_info = nullptr;
@@ -536,9 +543,10 @@ BinState bin_state(const mi355nufft_plan* p) {
     bs.bin_of = p->d_bin_i;
     bs.rank = p->d_bin_i + mc;
     bs.perm = p->d_bin_i + 2 * mc;
-    bs.counts = p->d_bin_b;
-    bs.bin_start = p->d_bin_b + bc;
-    bs.bin_sub = p->d_bin_b + 2 * bc;
+    bs.bkt = p->d_bin_b;  // first: its rows are read as int4
+    bs.counts = p->d_bin_b + kBanks * bc;
+    bs.bin_start = bs.counts + bc;
+    bs.bin_sub = bs.counts + 2 * bc;
     bs.sub_bin = p->d_sub;
     bs.sub_start = p->d_sub + sc;
     bs.n_sub = p->d_sub + 2 * sc;
@@ -578,22 +586,26 @@ int bin_points_impl(mi355nufft_plan* p) {
     const int64_t M = p->M;
     const int64_t max_sub = max_subproblems(p);
     RET_IF(grow(p, &p->d_bin_i, p->bin_m_cap, M, 3));
-    RET_IF(grow(p, &p->d_bin_b, p->bin_b_cap, (int64_t)p->bg.nbins, 3));
+    if ((int64_t)p->bg.nbins * kBanks >= ((int64_t)1 << 31))
+        return fail(MI355NUFFT_EINVAL, "too many bins (%d) for the bin sort", p->bg.nbins);
+    RET_IF(grow(p, &p->d_bin_b, p->bin_b_cap, (int64_t)p->bg.nbins, 3 + kBanks));
     RET_IF(grow(p, &p->d_sub, p->bin_s_cap, max_sub, 2));
     BinState bs = bin_state(p);
     InterpArgs<T, false> ia;
     int32_t J[3] = {p->J[0], p->J[1], p->J[2]};
     const void* dummy[3] = {p->d_tab[0], p->d_tab[0], p->d_tab[0]};
     fill_interp_args<T, false>(ia, p->ndim, p->K, J, p->L, dummy, p->d_tm, M, 1);
-    hipLaunchKernelGGL(fill_int_kernel, dim3(nblocks(p->bg.nbins)), dim3(kBlock), 0, p->stream, bs.counts,
-                       (int64_t)p->bg.nbins, 0);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(nblocks((int64_t)p->bg.nbins * kBanks)), dim3(kBlock), 0, p->stream,
+                       bs.bkt, (int64_t)p->bg.nbins * kBanks, 0);
     LAUNCH_CHECK();
     hipLaunchKernelGGL((bin_count_kernel<T>), dim3(nblocks(M)), dim3(kBlock), 0, p->stream, (const T*)p->d_tm, M,
                        p->ndim, ia.ax[0], ia.ax[1], ia.ax[2], p->bg, bs);
     LAUNCH_CHECK();
+    hipLaunchKernelGGL(bin_sum_kernel, dim3(nblocks(p->bg.nbins)), dim3(kBlock), 0, p->stream, bs, p->bg.nbins);
+    LAUNCH_CHECK();
     hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, p->stream, bs, p->bg.nbins, p->bg.P, (int)max_sub);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(bin_scatter_kernel, dim3(nblocks(M)), dim3(kBlock), 0, p->stream, M, bs);
+    hipLaunchKernelGGL(bin_scatter_kernel, dim3(nblocks(M)), dim3(kBlock), 0, p->stream, M, bs, p->bg.P);
     LAUNCH_CHECK();
     if (M > p->sorted_cap) {
         if (p->d_tms) {
@@ -617,6 +629,46 @@ int bin_points_impl(mi355nufft_plan* p) {
 }
 
 int bin_points(mi355nufft_plan* p) { return p->precision ? bin_points_impl<double>(p) : bin_points_impl<float>(p); }
+
+// Kernel timing: a start/stop event pair around each tiled gather or scatter
+// launch while timing is on and the stream is not being captured.
+constexpr int kTimingPairs = 1024;
+
+int timing_fold(mi355nufft_plan* p, int w) {
+    for (int i = 0; i < p->tev_used[w]; ++i) {
+        float ms = 0.f;
+        HIP_TRY(hipEventSynchronize(p->tev[w][2 * i + 1]));
+        HIP_TRY(hipEventElapsedTime(&ms, p->tev[w][2 * i], p->tev[w][2 * i + 1]));
+        p->tev_ms[w] += ms;
+        ++p->tev_n[w];
+    }
+    p->tev_used[w] = 0;
+    return MI355NUFFT_OK;
+}
+
+int timing_begin(mi355nufft_plan* p, int w, hipEvent_t* stop) {
+    *stop = nullptr;
+    if (!p->timing) return MI355NUFFT_OK;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(p->stream, &cs));
+    if (cs != hipStreamCaptureStatusNone) return MI355NUFFT_OK;
+    if (p->tev_used[w] == kTimingPairs) RET_IF(timing_fold(p, w));
+    const size_t need = 2 * (size_t)(p->tev_used[w] + 1);
+    while (p->tev[w].size() < need) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        p->tev[w].push_back(e);
+    }
+    const int i = p->tev_used[w]++;
+    HIP_TRY(hipEventRecord(p->tev[w][2 * i], p->stream));
+    *stop = p->tev[w][2 * i + 1];
+    return MI355NUFFT_OK;
+}
+
+int timing_end(mi355nufft_plan* p, hipEvent_t stop) {
+    if (stop) HIP_TRY(hipEventRecord(stop, p->stream));
+    return MI355NUFFT_OK;
+}
 
 template <typename T, bool CK, int NDIM, int JMAX, bool FIXED>
 void launch_tiled(bool adjoint, const TiledArgs<T, CK>& a, unsigned nsub, unsigned threads, size_t lds,
@@ -660,11 +712,13 @@ int run_tiled(mi355nufft_plan* p, bool adjoint, const void* samples_in, void* sa
     const size_t lds = adjoint ? tile_bytes(p) : gather_tile_bytes(p);
     const int jm = jmax_of(p->ndim, J);
     const int jf = fixed_j(p->ndim, J);
+    hipEvent_t stop = nullptr;
+    RET_IF(timing_begin(p, adjoint ? 1 : 0, &stop));
 #define MI355_FCASE(ND, JF)                                                    \
     if (p->ndim == ND && jf == JF) {                                           \
         launch_tiled<T, CK, ND, JF, true>(adjoint, a, nsub, p->bg_threads, lds, p->stream);   \
         LAUNCH_CHECK();                                                        \
-        return MI355NUFFT_OK;                                                  \
+        return timing_end(p, stop);                                            \
     }
     MI355_FCASE(1, 6) MI355_FCASE(2, 6) MI355_FCASE(3, 6)
 #undef MI355_FCASE
@@ -672,7 +726,7 @@ int run_tiled(mi355nufft_plan* p, bool adjoint, const void* samples_in, void* sa
     if (p->ndim == ND && jm == JM) {                                         \
         launch_tiled<T, CK, ND, JM, false>(adjoint, a, nsub, p->bg_threads, lds, p->stream); \
         LAUNCH_CHECK();                                                      \
-        return MI355NUFFT_OK;                                                \
+        return timing_end(p, stop);                                          \
     }
     MI355_TCASE(1, 4) MI355_TCASE(1, 8) MI355_TCASE(1, 16)
     MI355_TCASE(2, 4) MI355_TCASE(2, 8) MI355_TCASE(2, 16)
@@ -885,6 +939,8 @@ int mi355nufft_plan_destroy(mi355nufft_plan* p) {
     if (p->d_tiles) (void)hipFree(p->d_tiles);
     if (p->d_tms) (void)hipFree(p->d_tms);
     if (p->d_phs) (void)hipFree(p->d_phs);
+    for (auto& v : p->tev)
+        for (hipEvent_t e : v) (void)hipEventDestroy(e);
     delete p;
     return MI355NUFFT_OK;
 }
@@ -1007,6 +1063,23 @@ int mi355nufft_set_tiling(mi355nufft_plan* p, const int32_t* bin_size, int32_t m
         if (method == 4 && !p->reduce_ok)
             return fail(MI355NUFFT_EINVAL, "bins too small for the two-phase scatter");
     }
+    return MI355NUFFT_OK;
+}
+
+int mi355nufft_set_timing(mi355nufft_plan* p, int32_t enable) {
+    if (!p) return fail(MI355NUFFT_EINVAL, "null plan");
+    p->timing = enable ? 1 : 0;
+    return MI355NUFFT_OK;
+}
+
+int mi355nufft_read_timing(mi355nufft_plan* p, int32_t which, int64_t* launches, double* total_ms) {
+    if (!p || !launches || !total_ms) return fail(MI355NUFFT_EINVAL, "null argument");
+    if (which != 0 && which != 1) return fail(MI355NUFFT_EINVAL, "which must be 0 (gather) or 1 (scatter)");
+    RET_IF(timing_fold(p, which));
+    *launches = p->tev_n[which];
+    *total_ms = p->tev_ms[which];
+    p->tev_n[which] = 0;
+    p->tev_ms[which] = 0.0;
     return MI355NUFFT_OK;
 }
 

@@ -418,6 +418,41 @@ def test_graph_capture_plain_operator():
             _ = (x * 3).abs().sum().item()
 
 
+def test_kernel_timing_events():
+    """Event-timed tiled launches: counted while enabled, none while disabled
+    or while the stream is captured; reading resets the totals."""
+    Nd, Kd = (64, 64), (128, 128)
+    A = NufftBase(Nd, trj.golden_angle_2d(40, 128), Jd=6, Kd=Kd, Ld=1024, device=dev())
+    x = torch.from_numpy(trj.complex_gaussian(Nd, seed=31)).to(dev()).to(A._tcdt).unsqueeze(-1)
+    A._forward_dev(x, 1)
+    assert A.read_kernel_timing("interp")[0] == 0
+    A.set_kernel_timing(True)
+    for _ in range(3):
+        y = A._forward_dev(x, 1)
+        A._adjoint_dev(y, 1)
+    n_i, ms_i = A.read_kernel_timing("interp")
+    n_s, ms_s = A.read_kernel_timing("spread")
+    assert (n_i, n_s) == (3, 3)
+    assert 0.0 < ms_i < 100.0 and 0.0 < ms_s < 100.0
+    assert A.read_kernel_timing("spread")[0] == 0
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        A._forward_dev(x, 1)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    A.read_kernel_timing("interp")
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        A._forward_dev(x, 1)
+    g.replay()
+    torch.cuda.synchronize()
+    assert A.read_kernel_timing("interp")[0] == 0
+    A.set_kernel_timing(False)
+    with pytest.raises(KeyError):
+        A.read_kernel_timing("fft")
+
+
 @pytest.mark.parametrize("precision", ["single", "double"])
 @pytest.mark.parametrize("phasing", ["real", "complex"])
 def test_grid_only_and_psf_vs_oracle(precision, phasing):
