@@ -35,7 +35,7 @@ KERNEL_NAMES = {"interp": "interp_tiled_kernel<float, 2, false, 6, true>",
 # FETCH_SIZE and WRITE_SIZE passes over this bench (--calibrate-copy), each
 # scaled by what the counter reads for a 256 MiB device copy (FETCH_SIZE
 # x 2.0, WRITE_SIZE x 1.0); profiles/nufft2d_256_c64_hbm_traffic.md.
-TRAFFIC_BYTES = {}
+TRAFFIC_BYTES = {"spread": 20458188, "interp": 14012109}
 
 
 def parse():
@@ -133,12 +133,10 @@ class GpuRun:
     def stages(self):
         lib, p = self.lib, self.p
         return {
-            "deapod_pad": lambda: lib.mi355nufft_deapod_pad(p, self.px, self.xs, self.pg, 1),
-            "fft_forward": lambda: lib.mi355nufft_fft(p, self.pg, 1, -1),
+            "forward_grid": lambda: lib.mi355nufft_forward_grid(p, self.px, self.xs, self.pg, 1),
             "interp": lambda: lib.mi355nufft_interp(p, self.pg, self.py, 1),
             "spread": lambda: lib.mi355nufft_spread(p, self.py, self.pg, 1),
-            "fft_backward": lambda: lib.mi355nufft_fft(p, self.pg, 1, 1),
-            "crop_deapod": lambda: lib.mi355nufft_crop_deapod(p, self.pg, self.pxo, self.xos, 1),
+            "adjoint_image": lambda: lib.mi355nufft_adjoint_image(p, self.pg, self.pxo, self.xos, 1),
         }
 
     def time(self, fn, steps, warmup):
@@ -319,8 +317,9 @@ def main():
     stages = run.stage_times(max(10, args.steps // 2), 3)
     print("bench: stages (ms) %s" % json.dumps({k: round(v, 5) for k, v in stages.items()}),
           file=sys.stderr, flush=True)
-    # Kernel durations: a second eager run of the same steps with the plan's
-    # event pairs around every gather / scatter launch (on the plan's stream).
+    # Kernel durations: a second eager run of the same steps in which every
+    # gather / scatter launch carries a HIP event pair timestamped by its
+    # dispatch on the plan's stream (the kernel's execution only).
     run.A.set_kernel_timing(True)
     _time_fn(run.pair, args.steps, 0)
     run.A.set_kernel_timing(False)
@@ -339,6 +338,10 @@ def main():
                                  "kernel_ms": round(kern["interp" if dom == "spread" else "spread"][1], 5)}}
     extra = {}
     if not args.no_extra:
+        run.A.set_fft("rocfft")
+        t_roc = _time_fn(run.pair, args.steps, args.warmup)
+        run.A.set_fft("auto")
+        extra["c2_rocfft_pairs_per_s"] = round(1000.0 / t_roc, 2)
         r2 = GpuRun(c2_workload("double"))
         t2 = r2.time(r2.pair, max(5, args.steps // 2), 3)
         extra["c2_complex128_pairs_per_s"] = round(1000.0 / t2, 2)

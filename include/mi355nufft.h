@@ -149,8 +149,9 @@ int mi355nufft_set_tiling(mi355nufft_plan *plan, const int32_t *bin_size,
 
 /*
  * Kernel timing.  While enabled, every tiled gather (which = 0) or scatter
- * (which = 1) launch on a stream that is not being captured is bracketed by
- * a HIP event pair on the plan's stream.  mi355nufft_read_timing waits for
+ * (which = 1) launch on a stream that is not being captured carries a HIP
+ * event pair on the plan's stream, timestamped by the kernel's dispatch
+ * (hipExtLaunchKernel), so the pair spans the kernel's execution only.  mi355nufft_read_timing waits for
  * the recorded launches, returns their number and summed duration in
  * milliseconds, and resets both.  Instrumentation for benchmarks; no
  * counterpart in the reference.
@@ -163,7 +164,8 @@ int mi355nufft_read_timing(mi355nufft_plan *plan, int32_t which, int64_t *launch
  * Plan description: info[0..14] = ndim, M, bin size (3), points per
  * subproblem, number of bins, tile points, tiled kernels active (0/1),
  * subproblem slots, method, LDS budget in bytes, number of frames,
- * two-phase scatter active (0/1), threads per tiled workgroup.
+ * two-phase scatter active (0/1), threads per tiled workgroup, fused pruned
+ * FFT active (0/1).
  */
 int mi355nufft_plan_info(const mi355nufft_plan *plan, int64_t *info, int32_t n);
 
@@ -232,6 +234,37 @@ int mi355nufft_interp_grad(mi355nufft_plan *plan, const void *grid,
 /* Crop [:N] of the backward-FFT grid, scale, * conj(sn): _nufft.py:1561-1572. */
 int mi355nufft_crop_deapod(mi355nufft_plan *plan, const void *grid, void *x,
                            const int64_t *xs, int32_t nreps);
+
+/* ---- fused halves of the transforms ---- */
+
+/*
+ * Image -> oversampled grid, grid = fftn(x * sn, Kd) for nimg images (x and
+ * xs as mi355nufft_forward, nimg = frames * reps): _nufft.py:1326-1331.
+ * phase_before (:1371) is not applied here; the gather applies it.  Grids
+ * whose every K_d is a power of two in 16..2048 (16..1024 for complex128)
+ * run pruned FFT passes that
+ * read only the image (deapodization fused into the first pass); other
+ * grids run mi355nufft_deapod_pad + mi355nufft_fft.
+ */
+int mi355nufft_forward_grid(mi355nufft_plan *plan, const void *x,
+                            const int64_t *xs, void *grid, int32_t nimg);
+
+/*
+ * Oversampled grid -> image, x = ifftn(grid)[:Nd] * scale * conj(sn):
+ * _nufft.py:1524-1572 (conj(phase_before) of :1519 is applied by the
+ * scatter).  The grid is overwritten.  Power-of-two grids as above run
+ * pruned passes that write only the image rows (crop and deapodization
+ * fused into the last pass); others mi355nufft_fft + mi355nufft_crop_deapod.
+ */
+int mi355nufft_adjoint_image(mi355nufft_plan *plan, void *grid, void *x,
+                             const int64_t *xs, int32_t nimg);
+
+/*
+ * FFT used by mi355nufft_forward / adjoint / forward_grid / adjoint_image:
+ * method 0 auto (pruned fused passes where supported, else rocFFT), 1 rocFFT.
+ * No counterpart in the reference (which calls the array module's fftn).
+ */
+int mi355nufft_set_fft(mi355nufft_plan *plan, int method);
 
 /* ---- raw table1 kernels (no plan) ---- */
 

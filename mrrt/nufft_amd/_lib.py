@@ -57,6 +57,9 @@ _SIGNATURES = {
     "mi355nufft_spread": (c_int, [c_vp, c_vp, c_vp, c_i32]),
     "mi355nufft_interp_grad": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i32]),
     "mi355nufft_crop_deapod": (c_int, [c_vp, c_vp, c_vp, c_i64p, c_i32]),
+    "mi355nufft_forward_grid": (c_int, [c_vp, c_vp, c_i64p, c_vp, c_i32]),
+    "mi355nufft_adjoint_image": (c_int, [c_vp, c_vp, c_vp, c_i64p, c_i32]),
+    "mi355nufft_set_fft": (c_int, [c_vp, c_int]),
     "mi355nufft_table1_forward": (c_int, [c_int, c_i64p, c_i32p, c_i32, c_int, c_int, c_vp, c_vpp, c_vp, c_i64,
                                           c_vp, c_i32, c_vp]),
     "mi355nufft_table1_adjoint": (c_int, [c_int, c_i64p, c_i32p, c_i32, c_int, c_int, c_vp, c_vpp, c_vp, c_i64,

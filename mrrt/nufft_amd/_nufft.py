@@ -320,12 +320,19 @@ class NufftBase(object):
 
     def plan_info(self):
         """dict with the binning state of the plan (see mi355nufft_plan_info)."""
-        buf = (ctypes.c_int64 * 15)()
-        _lib.check(self._lib.mi355nufft_plan_info(self._plan, buf, 15))
+        buf = (ctypes.c_int64 * 16)()
+        _lib.check(self._lib.mi355nufft_plan_info(self._plan, buf, 16))
         v = list(buf)
         return dict(ndim=v[0], M=v[1], bin_size=tuple(v[2:5]), max_points=v[5], nbins=v[6], tile=v[7],
                     tiled=bool(v[8]), subproblem_slots=v[9], method=v[10], lds_limit=v[11], nframes=v[12],
-                    two_phase=bool(v[13]), threads=v[14])
+                    two_phase=bool(v[13]), threads=v[14], fused_fft=bool(v[15]))
+
+    def set_fft(self, method="auto"):
+        """FFT of the transforms: "auto" (pruned passes fused with the
+        deapodization for power-of-two grids, 16 <= K <= 2048, complex128
+        K <= 1024) or "rocfft"."""
+        m = {"auto": 0, "rocfft": 1}[method]
+        _lib.check(self._lib.mi355nufft_set_fft(self._plan, m))
 
     def set_kernel_timing(self, enable=True):
         """Bracket every tiled gather / scatter launch with HIP events on the

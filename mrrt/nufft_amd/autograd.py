@@ -52,14 +52,12 @@ def _ensure_points(op, omega, key):
 
 
 def _forward_grid(op, xb):
-    """deapod_pad + FFT of a (C, *Nd) batch; returns the (C, prod Kd) grid."""
+    """fftn(x * sn, Kd) of a (C, *Nd) batch; returns the (C, prod Kd) grid."""
     C = xb.shape[0]
     grid = torch.empty((C, int(np.prod(op.Kd))), dtype=op._tcdt, device=op.device)
     xv = _image_view(op, xb)
     op._sync_stream()
-    lib = op._lib
-    _lib.check(lib.mi355nufft_deapod_pad(op._plan, _vp(xv), _lib.i64arr(list(xv.stride())), _vp(grid), C))
-    _lib.check(lib.mi355nufft_fft(op._plan, _vp(grid), C, -1))
+    _lib.check(op._lib.mi355nufft_forward_grid(op._plan, _vp(xv), _lib.i64arr(list(xv.stride())), _vp(grid), C))
     return grid
 
 

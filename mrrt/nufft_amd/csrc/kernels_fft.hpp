@@ -1,0 +1,302 @@
+// Power-of-two FFTs of the oversampled grid, pruned to the zero-padded image
+// and fused with the deapodization.
+//
+// The forward transform of the table1 NUFFT is fftn(x * sn, Kd): only the
+// first N_d entries of every axis are non-zero before the FFT.  The adjoint
+// keeps only the first N_d entries of every axis after the inverse FFT.
+// Transforming axis 0 first (forward) and last (adjoint), every pass works on
+// the lines whose higher axes index the image (n_e < N_e) and whose lower
+// axes are already (forward) or still (adjoint) full length:
+//   forward axis d: lines (k_0..k_{d-1} < K, n_{d+1}.. < N), read t < N_d
+//                   (the rest is zero), write t < K_d;
+//   adjoint axis d: the same lines, read t < K_d, write t < N_d.
+// Axis 0 reads the image (times sn) in the forward and writes the image
+// (times scale * sn) in the adjoint, so deapod_pad and crop_deapod vanish
+// into the FFT passes, and rows and planes outside the image are never
+// read or written.
+//
+// A workgroup transforms lpw lines in LDS, TL threads per line, with a
+// Stockham auto-sort FFT (radix-8 passes, a final radix-2 or radix-4 pass);
+// twiddles come from a table of exp(-2 pi i m / K) rounded once from double,
+// staged in LDS.
+#pragma once
+
+#include "device_common.hpp"
+#include "kernels_grid.hpp"
+
+namespace mi355 {
+
+constexpr int kFftMinK = 16;
+constexpr int kFftMaxK = 2048;
+constexpr int kFftLds = 65536;
+constexpr int kFftMaxThreads = 1024;
+
+// TL threads transform one line: one radix-8 butterfly each per pass up to
+// K = 512, more per thread beyond.  The lines per workgroup (LPW) are chosen
+// at launch; the LDS holds LPW padded lines and the twiddle table.
+template <typename T, int K>
+struct FftCfg {
+    static constexpr int CXS = 2 * (int)sizeof(T);
+    // a line holds K points at pad(t) = t + t / 32 (strided butterfly writes
+    // spread over the banks), plus one so neighbouring lines start on
+    // different banks
+    static constexpr int KP = K + K / 32 + 1;
+    static constexpr int TL = K / 8 < 64 ? K / 8 : 64;
+    static constexpr size_t lds_bytes(int lpw) { return (size_t)(lpw * KP + K) * CXS; }
+};
+
+__device__ __forceinline__ int pad(int t) { return t + (t >> 5); }
+
+// Small DFTs: X[k] = sum_n x[n] exp(SIGN * 2 pi i n k / R)
+template <typename T, int SIGN>
+__device__ __forceinline__ cx<T> mul_i(cx<T> a) {  // a * (SIGN i)
+    return SIGN < 0 ? mkcx<T>(a.y, -a.x) : mkcx<T>(-a.y, a.x);
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> cadd(cx<T> a, cx<T> b) { return mkcx<T>(a.x + b.x, a.y + b.y); }
+template <typename T>
+__device__ __forceinline__ cx<T> csub(cx<T> a, cx<T> b) { return mkcx<T>(a.x - b.x, a.y - b.y); }
+template <typename T>
+__device__ __forceinline__ cx<T> cmulc(cx<T> a, cx<T> w) {
+    return mkcx<T>(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
+}
+
+template <typename T, int SIGN>
+__device__ __forceinline__ void dft2(cx<T>* v) {
+    const cx<T> a = v[0], b = v[1];
+    v[0] = cadd<T>(a, b);
+    v[1] = csub<T>(a, b);
+}
+
+template <typename T, int SIGN>
+__device__ __forceinline__ void dft4(cx<T>* v) {
+    const cx<T> c0 = cadd<T>(v[0], v[2]), c1 = csub<T>(v[0], v[2]);
+    const cx<T> d0 = cadd<T>(v[1], v[3]), d1 = mul_i<T, SIGN>(csub<T>(v[1], v[3]));
+    v[0] = cadd<T>(c0, d0);
+    v[1] = cadd<T>(c1, d1);
+    v[2] = csub<T>(c0, d0);
+    v[3] = csub<T>(c1, d1);
+}
+
+// radix 8 as two radix-4 DFTs of the even and odd halves after one radix-2
+// step with the twiddles exp(SIGN 2 pi i n / 8), n = 0..3
+template <typename T, int SIGN>
+__device__ __forceinline__ void dft8(cx<T>* v) {
+    const T h = (T)0.70710678118654752440;
+    cx<T> a[4], b[4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        a[n] = cadd<T>(v[n], v[n + 4]);
+        b[n] = csub<T>(v[n], v[n + 4]);
+    }
+    // b1 *= (h, SIGN h); b2 *= SIGN i; b3 *= (-h, SIGN h)
+    b[1] = mkcx<T>(h * (b[1].x - (T)SIGN * b[1].y), h * (b[1].y + (T)SIGN * b[1].x));
+    b[2] = mul_i<T, SIGN>(b[2]);
+    b[3] = mkcx<T>(-h * (b[3].x + (T)SIGN * b[3].y), h * ((T)SIGN * b[3].x - b[3].y));
+    dft4<T, SIGN>(a);
+    dft4<T, SIGN>(b);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v[2 * k] = a[k];
+        v[2 * k + 1] = b[k];
+    }
+}
+
+template <typename T, int R, int SIGN>
+__device__ __forceinline__ void dft_r(cx<T>* v) {
+    if constexpr (R == 8) dft8<T, SIGN>(v);
+    else if constexpr (R == 4) dft4<T, SIGN>(v);
+    else dft2<T, SIGN>(v);
+}
+
+// One Stockham pass of radix R over a line of K points in LDS, sub-transform
+// size NS before the pass: butterfly j reads j + r K/R, multiplies by
+// w^(k r K/(NS R)) with k = j mod NS, and writes (j / NS) NS R + k + r NS.
+// Every thread of the workgroup calls it (it holds the barriers).
+template <typename T, int K, int TL, int R, int NS, int SIGN>
+__device__ __forceinline__ void stockham_pass(cx<T>* line, int tl, const cx<T>* __restrict__ tw) {
+    constexpr int NBF = K / R;                      // butterflies per line
+    constexpr int NB = (NBF + TL - 1) / TL;         // per thread
+    cx<T> v[NB][R];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const int j = tl + q * TL;
+        if (j < NBF) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q][r] = line[pad(j + r * NBF)];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const int j = tl + q * TL;
+        if (j < NBF) {
+            const int k = j % NS;
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    cx<T> w = tw[k * r * (K / (NS * R))];
+                    if (SIGN > 0) w.y = -w.y;
+                    v[q][r] = cmulc<T>(v[q][r], w);
+                }
+            }
+            dft_r<T, R, SIGN>(v[q]);
+            const int o = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) line[pad(o + r * NS)] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int K, int TL, int NS, int SIGN>
+__device__ __forceinline__ void stockham_from(cx<T>* line, int tl, const cx<T>* __restrict__ tw) {
+    if constexpr (NS < K) {
+        constexpr int rest = K / NS;
+        constexpr int R = rest >= 8 ? 8 : rest;  // radix 8, the last pass 2 or 4
+        stockham_pass<T, K, TL, R, NS, SIGN>(line, tl, tw);
+        stockham_from<T, K, TL, NS * R, SIGN>(line, tl, tw);
+    }
+}
+
+// in-place FFT of one LDS line by TL threads (thread tl of the line)
+template <typename T, int K, int TL, int SIGN>
+__device__ __forceinline__ void fft_line(cx<T>* line, int tl, const cx<T>* __restrict__ tw) {
+    stockham_from<T, K, TL, 1, SIGN>(line, tl, tw);
+}
+
+struct FftRowGeom {
+    int N[3];          // image size
+    int K[3];          // grid size
+    int64_t nk;        // prod K
+    int64_t xs[4];     // image strides: axes 0..2, repetition
+    int64_t nlines;    // N1 * N2 * nimg
+    int lpw;           // lines per workgroup (blockDim = lpw * TL)
+};
+
+template <typename T, int K>
+__device__ __forceinline__ cx<T>* stage_twiddles(unsigned char* smem, int lpw, const cx<T>* __restrict__ tw) {
+    cx<T>* tws = reinterpret_cast<cx<T>*>(smem) + lpw * FftCfg<T, K>::KP;
+    for (int i = threadIdx.x; i < K; i += blockDim.x) tws[i] = tw[i];
+    return tws;
+}
+
+__device__ __forceinline__ void row_of_line(int64_t line, const FftRowGeom& g, int& n2, int& n3, int64_t& r) {
+    int64_t q = line;
+    n2 = (int)(q % g.N[1]);
+    q /= g.N[1];
+    n3 = (int)(q % g.N[2]);
+    r = q / g.N[2];
+}
+
+// Axis 0, forward: grid[r, n2, n3, :] = FFT_K0(pad(x[:, n2, n3, r] * sn)) for
+// n2 < N1, n3 < N2 (sn as deapod_factor).
+template <typename T, int K, int SIGN>
+__global__ __launch_bounds__(kFftMaxThreads) void fft_rows_pad_kernel(const cx<T>* __restrict__ x,
+                                                                      cx<T>* __restrict__ grid, FftRowGeom g,
+                                                                      const double* sn0, const double* sn1,
+                                                                      const double* sn2, int ndim,
+                                                                      const cx<T>* __restrict__ tw) {
+    using C = FftCfg<T, K>;
+    constexpr int TL = C::TL;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int li = threadIdx.x / TL, tl = threadIdx.x % TL;
+    const int64_t line = (int64_t)blockIdx.x * g.lpw + li;
+    const bool live = line < g.nlines;
+    int n2 = 0, n3 = 0;
+    int64_t r = 0;
+    if (live) row_of_line(line, g, n2, n3, r);
+    cx<T>* ln = reinterpret_cast<cx<T>*>(smem) + li * C::KP;
+    const double* sn[3] = {sn0, sn1, sn2};
+    for (int t = tl; t < K; t += TL) {
+        cx<T> o = mkcx<T>((T)0, (T)0);
+        if (live && t < g.N[0]) {
+            const T s = deapod_factor<T>(sn, ndim, t, n2, n3);
+            const cx<T> v = x[(int64_t)t * g.xs[0] + (int64_t)n2 * g.xs[1] + (int64_t)n3 * g.xs[2] + r * g.xs[3]];
+            o.x = v.x * s;
+            o.y = v.y * s;
+        }
+        ln[pad(t)] = o;
+    }
+    cx<T>* tws = stage_twiddles<T, K>(smem, g.lpw, tw);  // after the data loads are in flight
+    __syncthreads();
+    fft_line<T, K, TL, SIGN>(ln, tl, tws);
+    if (live) {
+        cx<T>* out = grid + r * g.nk + ((int64_t)n3 * g.K[1] + n2) * K;
+        for (int t = tl; t < K; t += TL) out[t] = ln[pad(t)];
+    }
+}
+
+// Axis 0, adjoint: x[:N0, n2, n3, r] = IFFT_K0(grid[r, n2, n3, :])[:N0] *
+// scale * sn for n2 < N1, n3 < N2 (the arithmetic of crop_deapod_kernel).
+template <typename T, int K, int SIGN>
+__global__ __launch_bounds__(kFftMaxThreads) void fft_rows_crop_kernel(const cx<T>* __restrict__ grid,
+                                                                       cx<T>* __restrict__ x, FftRowGeom g,
+                                                                       const double* sn0, const double* sn1,
+                                                                       const double* sn2, int ndim, T scale,
+                                                                       const cx<T>* __restrict__ tw) {
+    using C = FftCfg<T, K>;
+    constexpr int TL = C::TL;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int li = threadIdx.x / TL, tl = threadIdx.x % TL;
+    const int64_t line = (int64_t)blockIdx.x * g.lpw + li;
+    const bool live = line < g.nlines;
+    int n2 = 0, n3 = 0;
+    int64_t r = 0;
+    if (live) row_of_line(line, g, n2, n3, r);
+    cx<T>* ln = reinterpret_cast<cx<T>*>(smem) + li * C::KP;
+    const cx<T>* in = grid + r * g.nk + ((int64_t)n3 * g.K[1] + n2) * K;
+    for (int t = tl; t < K; t += TL) ln[pad(t)] = live ? in[t] : mkcx<T>((T)0, (T)0);
+    cx<T>* tws = stage_twiddles<T, K>(smem, g.lpw, tw);
+    __syncthreads();
+    fft_line<T, K, TL, SIGN>(ln, tl, tws);
+    if (live) {
+        const double* sn[3] = {sn0, sn1, sn2};
+        for (int t = tl; t < g.N[0]; t += TL) {
+            const T s = deapod_factor<T>(sn, ndim, t, n2, n3);
+            const cx<T> v = ln[pad(t)];
+            cx<T> o;
+            o.x = (v.x * scale) * s;
+            o.y = (v.y * scale) * s;
+            x[(int64_t)t * g.xs[0] + (int64_t)n2 * g.xs[1] + (int64_t)n3 * g.xs[2] + r * g.xs[3]] = o;
+        }
+    }
+}
+
+struct FftColGeom {
+    int64_t S;        // stride of the axis = number of lines side by side (prod of lower K)
+    int n_in;         // points read per line (the rest is zero)
+    int n_out;        // points written per line
+    int A;            // active indices of the next higher axis (1 if none)
+    int64_t st_a;     // grid stride of that axis
+    int64_t nk;       // prod K (stride of an image)
+    int64_t nouter;   // A * nimg
+    int lpw;          // neighbouring lines per workgroup, divides S (blockDim = lpw * TL)
+};
+
+// Axis d >= 1 in place: lpw neighbouring lines (inner index i0..i0+lpw-1) of
+// one outer index per workgroup; each row of the block is one contiguous
+// lpw-element segment.
+template <typename T, int K, int SIGN>
+__global__ __launch_bounds__(kFftMaxThreads) void fft_cols_kernel(cx<T>* __restrict__ grid, FftColGeom g,
+                                                                  const cx<T>* __restrict__ tw) {
+    using C = FftCfg<T, K>;
+    constexpr int TL = C::TL;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
+    const int lpw = g.lpw;
+    const int64_t nchunk = g.S / lpw;
+    const int64_t chunk = blockIdx.x % nchunk;
+    const int64_t o = blockIdx.x / nchunk;
+    const int64_t base = (o % g.A) * g.st_a + (o / g.A) * g.nk + chunk * lpw;
+    const int li = threadIdx.x % lpw, t0 = threadIdx.x / lpw;  // load / store mapping
+    for (int t = t0; t < K; t += TL)
+        lds[li * C::KP + pad(t)] = t < g.n_in ? grid[base + (int64_t)t * g.S + li] : mkcx<T>((T)0, (T)0);
+    cx<T>* tws = stage_twiddles<T, K>(smem, g.lpw, tw);
+    __syncthreads();
+    fft_line<T, K, TL, SIGN>(lds + (threadIdx.x / TL) * C::KP, threadIdx.x % TL, tws);
+    for (int t = t0; t < g.n_out; t += TL) grid[base + (int64_t)t * g.S + li] = lds[li * C::KP + pad(t)];
+}
+
+}  // namespace mi355

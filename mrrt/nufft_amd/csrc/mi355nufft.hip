@@ -3,6 +3,7 @@
 // deapodization factors, the per-sample state (tm, phase), the rocFFT plans
 // with their work buffer and an optional grid scratch buffer.  Everything is
 // enqueued on the plan's stream.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <rocfft/rocfft.h>
 
@@ -17,6 +18,7 @@
 #include "kernels_grid.hpp"
 #include "kernels_interp.hpp"
 #include "kernels_binned.hpp"
+#include "kernels_fft.hpp"
 
 using namespace mi355;
 
@@ -79,6 +81,9 @@ struct mi355nufft_plan {
     hipStream_t stream = nullptr;
 
     void* d_tab[3] = {nullptr, nullptr, nullptr};  // full tables (J*L+1 entries)
+    void* d_tw[3] = {nullptr, nullptr, nullptr};   // FFT twiddles exp(-2 pi i m / K_d), complex T
+    int fft_fused_ok = 0;  // every K_d a power of two in [kFftMinK, kFftMaxK]
+    int fft_method = 0;    // 0 auto (fused pruned FFT when possible), 1 rocFFT
     double* d_sn[3] = {nullptr, nullptr, nullptr};
 
     int64_t M = 0;
@@ -406,6 +411,151 @@ int crop_deapod_impl(mi355nufft_plan* p, const void* grid, void* x, const int64_
     return MI355NUFFT_OK;
 }
 
+int fft_impl(mi355nufft_plan* p, void* grid, int32_t nreps, int direction);
+
+// Fused pruned FFT passes (kernels_fft.hpp) for power-of-two grids.
+bool use_fused_fft(const mi355nufft_plan* p) { return p->fft_method == 0 && p->fft_fused_ok; }
+
+FftRowGeom row_geom(const mi355nufft_plan* p, const int64_t* xs, int32_t nimg) {
+    FftRowGeom g;
+    for (int d = 0; d < 3; ++d) {
+        g.N[d] = (int)p->N[d];
+        g.K[d] = (int)p->K[d];
+    }
+    g.nk = p->nk;
+    for (int d = 0; d < 4; ++d) g.xs[d] = 0;
+    for (int d = 0; d < p->ndim; ++d) g.xs[d] = xs[d];
+    g.xs[3] = xs[p->ndim];
+    g.nlines = (int64_t)g.N[1] * g.N[2] * nimg;
+    return g;
+}
+
+// lines of axis d >= 1: LPW neighbouring lines of prod(K_0..K_{d-1}) per
+// workgroup, for every active index of the next axis and every image
+FftColGeom col_geom(const mi355nufft_plan* p, int d, bool adjoint, int32_t nimg) {
+    FftColGeom g;
+    g.S = 1;
+    for (int e = 0; e < d; ++e) g.S *= p->K[e];
+    g.n_in = (int)(adjoint ? p->K[d] : p->N[d]);
+    g.n_out = (int)(adjoint ? p->N[d] : p->K[d]);
+    g.A = d + 1 < p->ndim ? (int)p->N[d + 1] : 1;
+    g.st_a = g.S * p->K[d];
+    g.nk = p->nk;
+    g.nouter = (int64_t)g.A * nimg;
+    return g;
+}
+
+// Workgroups a pass aims for (measured best on 512^2 grids): lines per workgroup grow only
+// while the pass keeps at least this many workgroups.
+constexpr int64_t kFftTargetBlocks = 128;
+
+template <typename T, int K>
+int launch_fft_rows(mi355nufft_plan* p, bool adjoint, const void* x, void* xo, const int64_t* xs, void* grid,
+                    int32_t nimg, T scale) {
+    using C = FftCfg<T, K>;
+    FftRowGeom g = row_geom(p, xs, nimg);
+    int lpw = 1;
+    while (2 * lpw * C::TL <= 256 && C::lds_bytes(2 * lpw) <= (size_t)kFftLds &&
+           g.nlines / (2 * lpw) >= kFftTargetBlocks)
+        lpw *= 2;
+    g.lpw = lpw;
+    const int64_t nb = (g.nlines + lpw - 1) / lpw;
+    const size_t lds = C::lds_bytes(lpw);
+    const double* s1 = p->ndim > 1 ? p->d_sn[1] : p->d_sn[0];
+    const double* s2 = p->ndim > 2 ? p->d_sn[2] : p->d_sn[0];
+    if (adjoint)
+        hipLaunchKernelGGL((fft_rows_crop_kernel<T, K, 1>), dim3((unsigned)nb), dim3(lpw * C::TL), lds, p->stream,
+                           (const cx<T>*)grid, (cx<T>*)xo, g, p->d_sn[0], s1, s2, p->ndim, scale,
+                           (const cx<T>*)p->d_tw[0]);
+    else
+        hipLaunchKernelGGL((fft_rows_pad_kernel<T, K, -1>), dim3((unsigned)nb), dim3(lpw * C::TL), lds, p->stream,
+                           (const cx<T>*)x, (cx<T>*)grid, g, p->d_sn[0], s1, s2, p->ndim,
+                           (const cx<T>*)p->d_tw[0]);
+    LAUNCH_CHECK();
+    return MI355NUFFT_OK;
+}
+
+template <typename T, int K>
+int launch_fft_cols(mi355nufft_plan* p, bool adjoint, int d, void* grid, int32_t nimg) {
+    using C = FftCfg<T, K>;
+    FftColGeom g = col_geom(p, d, adjoint, nimg);
+    // up to 128 contiguous bytes per row, while the pass keeps its workgroups
+    int lpw = 1;
+    while (2 * lpw * C::CXS <= 128 && g.S % (2 * lpw) == 0 && 2 * lpw * C::TL <= kFftMaxThreads &&
+           C::lds_bytes(2 * lpw) <= (size_t)kFftLds && (g.S / (2 * lpw)) * g.nouter >= kFftTargetBlocks)
+        lpw *= 2;
+    g.lpw = lpw;
+    const int64_t nb = (g.S / lpw) * g.nouter;
+    if (nb >= ((int64_t)1 << 32)) return fail(MI355NUFFT_EINVAL, "FFT launch too large");
+    const size_t lds = C::lds_bytes(lpw);
+    if (adjoint)
+        hipLaunchKernelGGL((fft_cols_kernel<T, K, 1>), dim3((unsigned)nb), dim3(lpw * C::TL), lds, p->stream,
+                           (cx<T>*)grid, g, (const cx<T>*)p->d_tw[d]);
+    else
+        hipLaunchKernelGGL((fft_cols_kernel<T, K, -1>), dim3((unsigned)nb), dim3(lpw * C::TL), lds, p->stream,
+                           (cx<T>*)grid, g, (const cx<T>*)p->d_tw[d]);
+    LAUNCH_CHECK();
+    return MI355NUFFT_OK;
+}
+
+#define MI355_FFT_SWITCH(KV, CALL)                                        \
+    switch (KV) {                                                         \
+        case 16: return CALL(16);                                         \
+        case 32: return CALL(32);                                         \
+        case 64: return CALL(64);                                         \
+        case 128: return CALL(128);                                       \
+        case 256: return CALL(256);                                       \
+        case 512: return CALL(512);                                       \
+        case 1024: return CALL(1024);                                     \
+        case 2048: return CALL(2048);                                     \
+        default: return fail(MI355NUFFT_EINVAL, "no fused FFT for K=%d", (int)(KV)); \
+    }
+
+template <typename T>
+int fft_rows(mi355nufft_plan* p, bool adjoint, const void* x, void* xo, const int64_t* xs, void* grid,
+             int32_t nimg, T scale) {
+#define MI355_ROWS(KK) launch_fft_rows<T, KK>(p, adjoint, x, xo, xs, grid, nimg, scale)
+    MI355_FFT_SWITCH((int)p->K[0], MI355_ROWS)
+#undef MI355_ROWS
+}
+
+template <typename T>
+int fft_cols(mi355nufft_plan* p, bool adjoint, int d, void* grid, int32_t nimg) {
+#define MI355_COLS(KK) launch_fft_cols<T, KK>(p, adjoint, d, grid, nimg)
+    MI355_FFT_SWITCH((int)p->K[d], MI355_COLS)
+#undef MI355_COLS
+}
+
+double adjoint_image_scale(const mi355nufft_plan* p) {
+    return p->ortho ? p->scale_ortho * p->adjoint_scale / (double)p->nk : p->adjoint_scale;
+}
+
+// image -> oversampled grid: fftn(x * sn, Kd) (_nufft.py:1326-1331); the
+// phase_before of :1371 is applied by the gather
+template <typename T>
+int forward_grid_impl(mi355nufft_plan* p, const void* x, const int64_t* xs, void* grid, int32_t nimg) {
+    if (!use_fused_fft(p)) {
+        RET_IF(deapod_pad_impl<T>(p, x, xs, grid, nimg));
+        return fft_impl(p, grid, nimg, -1);
+    }
+    RET_IF(fft_rows<T>(p, false, x, nullptr, xs, grid, nimg, (T)1));
+    for (int d = 1; d < p->ndim; ++d) RET_IF(fft_cols<T>(p, false, d, grid, nimg));
+    return MI355NUFFT_OK;
+}
+
+// oversampled grid -> image: ifftn, crop to Nd, scale, * conj(sn)
+// (_nufft.py:1524-1572); the conj(phase_before) of :1519 is applied by the
+// scatter.  The grid is overwritten.
+template <typename T>
+int adjoint_image_impl(mi355nufft_plan* p, void* grid, void* x, const int64_t* xs, int32_t nimg) {
+    if (!use_fused_fft(p)) {
+        RET_IF(fft_impl(p, grid, nimg, 1));
+        return crop_deapod_impl<T>(p, grid, x, xs, nimg);
+    }
+    for (int d = p->ndim - 1; d >= 1; --d) RET_IF(fft_cols<T>(p, true, d, grid, nimg));
+    return fft_rows<T>(p, true, nullptr, x, xs, grid, nimg, (T)adjoint_image_scale(p));
+}
+
 template <typename T>
 int run_tiled_any(mi355nufft_plan* p, bool adjoint, const void* samples_in, void* samples_out, const void* grid_in,
                   void* grid_out, int32_t nreps);
@@ -630,8 +780,10 @@ int bin_points_impl(mi355nufft_plan* p) {
 
 int bin_points(mi355nufft_plan* p) { return p->precision ? bin_points_impl<double>(p) : bin_points_impl<float>(p); }
 
-// Kernel timing: a start/stop event pair around each tiled gather or scatter
-// launch while timing is on and the stream is not being captured.
+// Kernel timing: while timing is on and the stream is not being captured,
+// each tiled gather or scatter launch goes through hipExtLaunchKernel with a
+// start/stop event pair that the dispatch itself timestamps (the kernel's own
+// execution, not the gap before it).
 constexpr int kTimingPairs = 1024;
 
 int timing_fold(mi355nufft_plan* p, int w) {
@@ -646,8 +798,9 @@ int timing_fold(mi355nufft_plan* p, int w) {
     return MI355NUFFT_OK;
 }
 
-int timing_begin(mi355nufft_plan* p, int w, hipEvent_t* stop) {
-    *stop = nullptr;
+// events for the next launch of kind w, or nulls when not timing
+int timing_events(mi355nufft_plan* p, int w, hipEvent_t* start, hipEvent_t* stop) {
+    *start = *stop = nullptr;
     if (!p->timing) return MI355NUFFT_OK;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     HIP_TRY(hipStreamIsCapturing(p->stream, &cs));
@@ -660,19 +813,23 @@ int timing_begin(mi355nufft_plan* p, int w, hipEvent_t* stop) {
         p->tev[w].push_back(e);
     }
     const int i = p->tev_used[w]++;
-    HIP_TRY(hipEventRecord(p->tev[w][2 * i], p->stream));
+    *start = p->tev[w][2 * i];
     *stop = p->tev[w][2 * i + 1];
-    return MI355NUFFT_OK;
-}
-
-int timing_end(mi355nufft_plan* p, hipEvent_t stop) {
-    if (stop) HIP_TRY(hipEventRecord(stop, p->stream));
     return MI355NUFFT_OK;
 }
 
 template <typename T, bool CK, int NDIM, int JMAX, bool FIXED>
 void launch_tiled(bool adjoint, const TiledArgs<T, CK>& a, unsigned nsub, unsigned threads, size_t lds,
-                  hipStream_t s) {
+                  hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+    if (ev0) {
+        if (adjoint)
+            hipExtLaunchKernelGGL((spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED>), dim3(nsub), dim3(threads),
+                                  (uint32_t)lds, s, ev0, ev1, 0u, a);
+        else
+            hipExtLaunchKernelGGL((interp_tiled_kernel<T, NDIM, CK, JMAX, FIXED>), dim3(nsub), dim3(threads),
+                                  (uint32_t)lds, s, ev0, ev1, 0u, a);
+        return;
+    }
     if (adjoint)
         hipLaunchKernelGGL((spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED>), dim3(nsub), dim3(threads), lds, s, a);
     else
@@ -712,21 +869,21 @@ int run_tiled(mi355nufft_plan* p, bool adjoint, const void* samples_in, void* sa
     const size_t lds = adjoint ? tile_bytes(p) : gather_tile_bytes(p);
     const int jm = jmax_of(p->ndim, J);
     const int jf = fixed_j(p->ndim, J);
-    hipEvent_t stop = nullptr;
-    RET_IF(timing_begin(p, adjoint ? 1 : 0, &stop));
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    RET_IF(timing_events(p, adjoint ? 1 : 0, &ev0, &ev1));
 #define MI355_FCASE(ND, JF)                                                    \
     if (p->ndim == ND && jf == JF) {                                           \
-        launch_tiled<T, CK, ND, JF, true>(adjoint, a, nsub, p->bg_threads, lds, p->stream);   \
+        launch_tiled<T, CK, ND, JF, true>(adjoint, a, nsub, p->bg_threads, lds, p->stream, ev0, ev1); \
         LAUNCH_CHECK();                                                        \
-        return timing_end(p, stop);                                            \
+        return MI355NUFFT_OK;                                                  \
     }
     MI355_FCASE(1, 6) MI355_FCASE(2, 6) MI355_FCASE(3, 6)
 #undef MI355_FCASE
 #define MI355_TCASE(ND, JM)                                                  \
     if (p->ndim == ND && jm == JM) {                                         \
-        launch_tiled<T, CK, ND, JM, false>(adjoint, a, nsub, p->bg_threads, lds, p->stream); \
+        launch_tiled<T, CK, ND, JM, false>(adjoint, a, nsub, p->bg_threads, lds, p->stream, ev0, ev1); \
         LAUNCH_CHECK();                                                      \
-        return timing_end(p, stop);                                          \
+        return MI355NUFFT_OK;                                                \
     }
     MI355_TCASE(1, 4) MI355_TCASE(1, 8) MI355_TCASE(1, 16)
     MI355_TCASE(2, 4) MI355_TCASE(2, 8) MI355_TCASE(2, 16)
@@ -912,6 +1069,38 @@ int mi355nufft_plan_create(mi355nufft_plan** plan, int ndim, const int64_t* Nd, 
             return fail(MI355NUFFT_EHIP, "table / sn upload");
         }
     }
+    // one padded line and the twiddle table must fit the LDS budget
+    // (complex128 lines of 2048 points do not)
+    p->fft_fused_ok = 1;
+    for (int d = 0; d < ndim; ++d) {
+        const int64_t k = Kd[d];
+        if (k < kFftMinK || k > kFftMaxK || (k & (k - 1)) != 0 ||
+            (size_t)(2 * k + k / 32 + 1) * p->cx_size() > (size_t)kFftLds)  // FftCfg::lds_bytes(1)
+            p->fft_fused_ok = 0;
+    }
+    if (p->fft_fused_ok) {
+        for (int d = 0; d < ndim; ++d) {
+            const int64_t k = Kd[d];
+            std::vector<double> w((size_t)(2 * k));
+            for (int64_t m = 0; m < k; ++m) {
+                const double a = 2.0 * M_PI * (double)m / (double)k;
+                w[(size_t)(2 * m)] = std::cos(a);
+                w[(size_t)(2 * m + 1)] = -std::sin(a);
+            }
+            const size_t bytes = (size_t)k * p->cx_size();
+            std::vector<float> wf;
+            const void* src = w.data();
+            if (!precision) {
+                wf.assign(w.begin(), w.end());
+                src = wf.data();
+            }
+            if (hipMalloc(&p->d_tw[d], bytes) != hipSuccess ||
+                hipMemcpy(p->d_tw[d], src, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+                mi355nufft_plan_destroy(p);
+                return fail(MI355NUFFT_ENOMEM, "FFT twiddle table");
+            }
+        }
+    }
     *plan = p;
     return MI355NUFFT_OK;
 }
@@ -927,6 +1116,7 @@ int mi355nufft_plan_destroy(mi355nufft_plan* p) {
     for (int d = 0; d < 3; ++d) {
         if (p->d_tab[d]) (void)hipFree(p->d_tab[d]);
         if (p->d_sn[d]) (void)hipFree(p->d_sn[d]);
+        if (p->d_tw[d]) (void)hipFree(p->d_tw[d]);
     }
     if (p->d_tm) (void)hipFree(p->d_tm);
     if (p->d_phase) (void)hipFree(p->d_phase);
@@ -1087,10 +1277,11 @@ int mi355nufft_plan_info(const mi355nufft_plan* p, int64_t* info, int32_t n) {
     g_err.clear();
     RET_IF(check_plan(p));
     if (!info || n < 1) return fail(MI355NUFFT_EINVAL, "info buffer required");
-    const int64_t v[15] = {p->ndim, p->M, p->bg.B[0], p->bg.B[1], p->bg.B[2], p->bg.P, p->bg.nbins, p->bg.tile,
+    const int64_t v[16] = {p->ndim, p->M, p->bg.B[0], p->bg.B[1], p->bg.B[2], p->bg.P, p->bg.nbins, p->bg.tile,
                            use_tiled(p) ? 1 : 0, p->binned ? max_subproblems(p) : 0, p->method, (int64_t)p->lds_limit,
-                           p->nframes, (use_tiled(p) && use_two_phase(p, 1)) ? 1 : 0, p->bg_threads};
-    for (int i = 0; i < n && i < 15; ++i) info[i] = v[i];
+                           p->nframes, (use_tiled(p) && use_two_phase(p, 1)) ? 1 : 0, p->bg_threads,
+                           use_fused_fft(p) ? 1 : 0};
+    for (int i = 0; i < n && i < 16; ++i) info[i] = v[i];
     return MI355NUFFT_OK;
 }
 
@@ -1155,6 +1346,32 @@ int mi355nufft_crop_deapod(mi355nufft_plan* p, const void* grid, void* x, const 
                         : crop_deapod_impl<float>(p, grid, x, xs, nreps);
 }
 
+int mi355nufft_forward_grid(mi355nufft_plan* p, const void* x, const int64_t* xs, void* grid, int32_t nimg) {
+    g_err.clear();
+    RET_IF(check_plan(p));
+    RET_IF(check_nreps(nimg));
+    if (!x || !xs || !grid) return fail(MI355NUFFT_EINVAL, "x, xs and grid are required");
+    return p->precision ? forward_grid_impl<double>(p, x, xs, grid, nimg)
+                        : forward_grid_impl<float>(p, x, xs, grid, nimg);
+}
+
+int mi355nufft_adjoint_image(mi355nufft_plan* p, void* grid, void* x, const int64_t* xs, int32_t nimg) {
+    g_err.clear();
+    RET_IF(check_plan(p));
+    RET_IF(check_nreps(nimg));
+    if (!x || !xs || !grid) return fail(MI355NUFFT_EINVAL, "grid, x and xs are required");
+    return p->precision ? adjoint_image_impl<double>(p, grid, x, xs, nimg)
+                        : adjoint_image_impl<float>(p, grid, x, xs, nimg);
+}
+
+int mi355nufft_set_fft(mi355nufft_plan* p, int method) {
+    g_err.clear();
+    RET_IF(check_plan(p));
+    if (method != 0 && method != 1) return fail(MI355NUFFT_EINVAL, "FFT method must be 0 (auto) or 1 (rocFFT)");
+    p->fft_method = method;
+    return MI355NUFFT_OK;
+}
+
 int mi355nufft_forward(mi355nufft_plan* p, const void* x, const int64_t* xs, void* y, int32_t nreps, void* grid) {
     g_err.clear();
     RET_IF(check_plan(p));
@@ -1164,8 +1381,7 @@ int mi355nufft_forward(mi355nufft_plan* p, const void* x, const int64_t* xs, voi
     const int32_t nimg = p->nframes * nreps;
     RET_IF(check_nreps(nimg));
     RET_IF(get_grid(p, nimg, &grid));
-    RET_IF(mi355nufft_deapod_pad(p, x, xs, grid, nimg));
-    RET_IF(mi355nufft_fft(p, grid, nimg, -1));
+    RET_IF(mi355nufft_forward_grid(p, x, xs, grid, nimg));
     return mi355nufft_interp(p, grid, y, nreps);
 }
 
@@ -1179,8 +1395,7 @@ int mi355nufft_adjoint(mi355nufft_plan* p, const void* y, void* x, const int64_t
     RET_IF(check_nreps(nimg));
     RET_IF(get_grid(p, nimg, &grid));
     RET_IF(mi355nufft_spread(p, y, grid, nreps));
-    RET_IF(mi355nufft_fft(p, grid, nimg, 1));
-    return mi355nufft_crop_deapod(p, grid, x, xs, nimg);
+    return mi355nufft_adjoint_image(p, grid, x, xs, nimg);
 }
 
 int mi355nufft_table1_forward(int ndim, const int64_t* Kd, const int32_t* Jd, int32_t L, int precision,
