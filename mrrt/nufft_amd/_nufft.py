@@ -429,11 +429,11 @@ class NufftBase(object):
         if out is None:
             out = torch.empty((nreps,) + tuple(reversed(self._Nd)), dtype=self._tcdt, device=self.device)
             out = out.permute(*(tuple(range(self.ndim, 0, -1)) + (0,)))
-        grid = torch.empty((self.nframes * nreps, int(np.prod(self._Kd))), dtype=self._tcdt, device=self.device)
         self._sync_stream()
+        # plan-owned grid: kept zero between adjoints by the fused FFT passes
         _lib.check(self._lib.mi355nufft_adjoint(
             self._plan, ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(out.data_ptr()),
-            _lib.i64arr(list(out.stride())), nreps, ctypes.c_void_p(grid.data_ptr())))
+            _lib.i64arr(list(out.stride())), nreps, None))
         return out
 
 

@@ -90,7 +90,7 @@ class NufftFrames(object):
         op._sync_stream()
         _lib.check(op._lib.mi355nufft_adjoint(op._plan, ctypes.c_void_p(y.data_ptr()),
                                               ctypes.c_void_p(xv.data_ptr()), _lib.i64arr(list(xv.stride())),
-                                              self.ncoils, ctypes.c_void_p(self._grid.data_ptr())))
+                                              self.ncoils, None))  # plan-owned, kept zero
         return x
 
     def capture(self, x, y, xa):

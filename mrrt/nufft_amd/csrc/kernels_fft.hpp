@@ -230,12 +230,14 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_rows_pad_kernel(const cx<T
 
 // Axis 0, adjoint: x[:N0, n2, n3, r] = IFFT_K0(grid[r, n2, n3, :])[:N0] *
 // scale * sn for n2 < N1, n3 < N2 (the arithmetic of crop_deapod_kernel).
+//   rezero: write zeros over the rows it read, so that a grid passed through
+//   the adjoint with zero_tail set on every other pass ends all zero.
 template <typename T, int K, int SIGN>
-__global__ __launch_bounds__(kFftMaxThreads) void fft_rows_crop_kernel(const cx<T>* __restrict__ grid,
+__global__ __launch_bounds__(kFftMaxThreads) void fft_rows_crop_kernel(cx<T>* __restrict__ grid,
                                                                        cx<T>* __restrict__ x, FftRowGeom g,
                                                                        const double* sn0, const double* sn1,
                                                                        const double* sn2, int ndim, T scale,
-                                                                       const cx<T>* __restrict__ tw) {
+                                                                       const cx<T>* __restrict__ tw, int rezero) {
     using C = FftCfg<T, K>;
     constexpr int TL = C::TL;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -246,10 +248,12 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_rows_crop_kernel(const cx<
     int64_t r = 0;
     if (live) row_of_line(line, g, n2, n3, r);
     cx<T>* ln = reinterpret_cast<cx<T>*>(smem) + li * C::KP;
-    const cx<T>* in = grid + r * g.nk + ((int64_t)n3 * g.K[1] + n2) * K;
+    cx<T>* in = grid + r * g.nk + ((int64_t)n3 * g.K[1] + n2) * K;
     for (int t = tl; t < K; t += TL) ln[pad(t)] = live ? in[t] : mkcx<T>((T)0, (T)0);
     cx<T>* tws = stage_twiddles<T, K>(smem, g.lpw, tw);
     __syncthreads();
+    if (rezero && live)  // each thread clears the points it loaded
+        for (int t = tl; t < K; t += TL) in[t] = mkcx<T>((T)0, (T)0);
     fft_line<T, K, TL, SIGN>(ln, tl, tws);
     if (live) {
         const double* sn[3] = {sn0, sn1, sn2};
@@ -273,6 +277,7 @@ struct FftColGeom {
     int64_t nk;       // prod K (stride of an image)
     int64_t nouter;   // A * nimg
     int lpw;          // neighbouring lines per workgroup, divides S (blockDim = lpw * TL)
+    int zero_tail;    // also write zeros at n_out <= t < K
 };
 
 // Axis d >= 1 in place: lpw neighbouring lines (inner index i0..i0+lpw-1) of
@@ -297,6 +302,8 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_cols_kernel(cx<T>* __restr
     __syncthreads();
     fft_line<T, K, TL, SIGN>(lds + (threadIdx.x / TL) * C::KP, threadIdx.x % TL, tws);
     for (int t = t0; t < g.n_out; t += TL) grid[base + (int64_t)t * g.S + li] = lds[li * C::KP + pad(t)];
+    if (g.zero_tail)
+        for (int t = g.n_out + t0; t < K; t += TL) grid[base + (int64_t)t * g.S + li] = mkcx<T>((T)0, (T)0);
 }
 
 }  // namespace mi355

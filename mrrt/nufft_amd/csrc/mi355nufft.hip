@@ -93,8 +93,14 @@ struct mi355nufft_plan {
     int has_phase = 0;
     double shift[3] = {0, 0, 0};  // exponent coefficients of the per-sample phase
 
-    void* d_grid = nullptr;  // plan-owned grid scratch
+    void* d_grid = nullptr;  // plan-owned grid scratch (forward)
     size_t grid_bytes = 0;
+    // plan-owned adjoint grid; the fused adjoint FFT passes write zeros over
+    // everything they read, so after an adjoint through it the buffer is all
+    // zero again and the next scatter needs no separate clear
+    void* d_agrid = nullptr;
+    size_t agrid_bytes = 0;
+    int agrid_zero = 0;  // d_agrid holds zeros (host-side state, stream order)
 
     // binning (tiled kernels)
     int method = 0;           // 0 auto, 1 global-memory, 2 tiled, 3 tiled + atomic flush, 4 tiled + two-phase
@@ -280,6 +286,29 @@ int check_nreps(int32_t nreps) {
         if (rc_) return rc_;      \
     } while (0)
 
+int get_agrid(mi355nufft_plan* p, int32_t nimg) {
+    const size_t need = (size_t)nimg * (size_t)p->nk * p->cx_size();
+    if (need > p->agrid_bytes) {
+        p->agrid_zero = 0;
+        if (p->d_agrid) {
+            HIP_TRY(hipStreamSynchronize(p->stream));
+            HIP_TRY(hipFree(p->d_agrid));
+            p->d_agrid = nullptr;
+            p->agrid_bytes = 0;
+        }
+        if (hipMalloc(&p->d_agrid, need) != hipSuccess)
+            return fail(MI355NUFFT_ENOMEM, "adjoint grid of %zu bytes", need);
+        p->agrid_bytes = need;
+    }
+    return MI355NUFFT_OK;
+}
+
+bool stream_capturing(const mi355nufft_plan* p) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(p->stream, &cs) != hipSuccess) return true;  // be conservative
+    return cs != hipStreamCaptureStatusNone;
+}
+
 int get_grid(mi355nufft_plan* p, int32_t nreps, void** grid) {
     if (*grid) return MI355NUFFT_OK;
     const size_t need = (size_t)nreps * (size_t)p->nk * p->cx_size();
@@ -451,7 +480,7 @@ constexpr int64_t kFftTargetBlocks = 128;
 
 template <typename T, int K>
 int launch_fft_rows(mi355nufft_plan* p, bool adjoint, const void* x, void* xo, const int64_t* xs, void* grid,
-                    int32_t nimg, T scale) {
+                    int32_t nimg, T scale, int rezero) {
     using C = FftCfg<T, K>;
     FftRowGeom g = row_geom(p, xs, nimg);
     int lpw = 1;
@@ -465,8 +494,8 @@ int launch_fft_rows(mi355nufft_plan* p, bool adjoint, const void* x, void* xo, c
     const double* s2 = p->ndim > 2 ? p->d_sn[2] : p->d_sn[0];
     if (adjoint)
         hipLaunchKernelGGL((fft_rows_crop_kernel<T, K, 1>), dim3((unsigned)nb), dim3(lpw * C::TL), lds, p->stream,
-                           (const cx<T>*)grid, (cx<T>*)xo, g, p->d_sn[0], s1, s2, p->ndim, scale,
-                           (const cx<T>*)p->d_tw[0]);
+                           (cx<T>*)grid, (cx<T>*)xo, g, p->d_sn[0], s1, s2, p->ndim, scale,
+                           (const cx<T>*)p->d_tw[0], rezero);
     else
         hipLaunchKernelGGL((fft_rows_pad_kernel<T, K, -1>), dim3((unsigned)nb), dim3(lpw * C::TL), lds, p->stream,
                            (const cx<T>*)x, (cx<T>*)grid, g, p->d_sn[0], s1, s2, p->ndim,
@@ -476,9 +505,10 @@ int launch_fft_rows(mi355nufft_plan* p, bool adjoint, const void* x, void* xo, c
 }
 
 template <typename T, int K>
-int launch_fft_cols(mi355nufft_plan* p, bool adjoint, int d, void* grid, int32_t nimg) {
+int launch_fft_cols(mi355nufft_plan* p, bool adjoint, int d, void* grid, int32_t nimg, int rezero) {
     using C = FftCfg<T, K>;
     FftColGeom g = col_geom(p, d, adjoint, nimg);
+    g.zero_tail = rezero;
     // up to 128 contiguous bytes per row, while the pass keeps its workgroups
     int lpw = 1;
     while (2 * lpw * C::CXS <= 128 && g.S % (2 * lpw) == 0 && 2 * lpw * C::TL <= kFftMaxThreads &&
@@ -513,15 +543,15 @@ int launch_fft_cols(mi355nufft_plan* p, bool adjoint, int d, void* grid, int32_t
 
 template <typename T>
 int fft_rows(mi355nufft_plan* p, bool adjoint, const void* x, void* xo, const int64_t* xs, void* grid,
-             int32_t nimg, T scale) {
-#define MI355_ROWS(KK) launch_fft_rows<T, KK>(p, adjoint, x, xo, xs, grid, nimg, scale)
+             int32_t nimg, T scale, int rezero) {
+#define MI355_ROWS(KK) launch_fft_rows<T, KK>(p, adjoint, x, xo, xs, grid, nimg, scale, rezero)
     MI355_FFT_SWITCH((int)p->K[0], MI355_ROWS)
 #undef MI355_ROWS
 }
 
 template <typename T>
-int fft_cols(mi355nufft_plan* p, bool adjoint, int d, void* grid, int32_t nimg) {
-#define MI355_COLS(KK) launch_fft_cols<T, KK>(p, adjoint, d, grid, nimg)
+int fft_cols(mi355nufft_plan* p, bool adjoint, int d, void* grid, int32_t nimg, int rezero) {
+#define MI355_COLS(KK) launch_fft_cols<T, KK>(p, adjoint, d, grid, nimg, rezero)
     MI355_FFT_SWITCH((int)p->K[d], MI355_COLS)
 #undef MI355_COLS
 }
@@ -538,22 +568,23 @@ int forward_grid_impl(mi355nufft_plan* p, const void* x, const int64_t* xs, void
         RET_IF(deapod_pad_impl<T>(p, x, xs, grid, nimg));
         return fft_impl(p, grid, nimg, -1);
     }
-    RET_IF(fft_rows<T>(p, false, x, nullptr, xs, grid, nimg, (T)1));
-    for (int d = 1; d < p->ndim; ++d) RET_IF(fft_cols<T>(p, false, d, grid, nimg));
+    RET_IF(fft_rows<T>(p, false, x, nullptr, xs, grid, nimg, (T)1, 0));
+    for (int d = 1; d < p->ndim; ++d) RET_IF(fft_cols<T>(p, false, d, grid, nimg, 0));
     return MI355NUFFT_OK;
 }
 
 // oversampled grid -> image: ifftn, crop to Nd, scale, * conj(sn)
 // (_nufft.py:1524-1572); the conj(phase_before) of :1519 is applied by the
-// scatter.  The grid is overwritten.
+// scatter.  The grid is overwritten; with rezero (fused passes only) it is
+// left all zero.
 template <typename T>
-int adjoint_image_impl(mi355nufft_plan* p, void* grid, void* x, const int64_t* xs, int32_t nimg) {
+int adjoint_image_impl(mi355nufft_plan* p, void* grid, void* x, const int64_t* xs, int32_t nimg, int rezero) {
     if (!use_fused_fft(p)) {
         RET_IF(fft_impl(p, grid, nimg, 1));
         return crop_deapod_impl<T>(p, grid, x, xs, nimg);
     }
-    for (int d = p->ndim - 1; d >= 1; --d) RET_IF(fft_cols<T>(p, true, d, grid, nimg));
-    return fft_rows<T>(p, true, nullptr, x, xs, grid, nimg, (T)adjoint_image_scale(p));
+    for (int d = p->ndim - 1; d >= 1; --d) RET_IF(fft_cols<T>(p, true, d, grid, nimg, rezero));
+    return fft_rows<T>(p, true, nullptr, x, xs, grid, nimg, (T)adjoint_image_scale(p), rezero);
 }
 
 template <typename T>
@@ -581,9 +612,9 @@ int spread_two_phase(mi355nufft_plan* p, const void* y, void* grid, int32_t nrep
 bool use_two_phase(const mi355nufft_plan* p, int32_t nreps);
 
 template <typename T>
-int spread_impl(mi355nufft_plan* p, const void* y, void* grid, int32_t nreps) {
+int spread_impl(mi355nufft_plan* p, const void* y, void* grid, int32_t nreps, bool grid_is_zero = false) {
     if (use_tiled(p) && use_two_phase(p, nreps)) return spread_two_phase<T>(p, y, grid, nreps);
-    RET_IF(zero_grid<T>(grid, (int64_t)p->nframes * nreps * p->nk, p->stream));
+    if (!grid_is_zero) RET_IF(zero_grid<T>(grid, (int64_t)p->nframes * nreps * p->nk, p->stream));
     if (use_tiled(p)) return run_tiled_any<T>(p, true, y, nullptr, nullptr, grid, nreps);
     if (p->nframes > 1) return fail(MI355NUFFT_ESTATE, "frames need the tiled kernels");
     int32_t J[3] = {p->J[0], p->J[1], p->J[2]};
@@ -1121,6 +1152,7 @@ int mi355nufft_plan_destroy(mi355nufft_plan* p) {
     if (p->d_tm) (void)hipFree(p->d_tm);
     if (p->d_phase) (void)hipFree(p->d_phase);
     if (p->d_grid) (void)hipFree(p->d_grid);
+    if (p->d_agrid) (void)hipFree(p->d_agrid);
     if (p->fft_work) (void)hipFree(p->fft_work);
     if (p->d_bin_i) (void)hipFree(p->d_bin_i);
     if (p->d_bin_b) (void)hipFree(p->d_bin_b);
@@ -1360,8 +1392,8 @@ int mi355nufft_adjoint_image(mi355nufft_plan* p, void* grid, void* x, const int6
     RET_IF(check_plan(p));
     RET_IF(check_nreps(nimg));
     if (!x || !xs || !grid) return fail(MI355NUFFT_EINVAL, "grid, x and xs are required");
-    return p->precision ? adjoint_image_impl<double>(p, grid, x, xs, nimg)
-                        : adjoint_image_impl<float>(p, grid, x, xs, nimg);
+    return p->precision ? adjoint_image_impl<double>(p, grid, x, xs, nimg, 0)
+                        : adjoint_image_impl<float>(p, grid, x, xs, nimg, 0);
 }
 
 int mi355nufft_set_fft(mi355nufft_plan* p, int method) {
@@ -1393,9 +1425,28 @@ int mi355nufft_adjoint(mi355nufft_plan* p, const void* y, void* x, const int64_t
     if (!x || !xs || (!y && p->M > 0)) return fail(MI355NUFFT_EINVAL, "y, x and xs are required");
     const int32_t nimg = p->nframes * nreps;
     RET_IF(check_nreps(nimg));
-    RET_IF(get_grid(p, nimg, &grid));
-    RET_IF(mi355nufft_spread(p, y, grid, nreps));
-    return mi355nufft_adjoint_image(p, grid, x, xs, nimg);
+    if (grid) {
+        RET_IF(mi355nufft_spread(p, y, grid, nreps));
+        return mi355nufft_adjoint_image(p, grid, x, xs, nimg);
+    }
+    // plan-owned adjoint grid, kept zero between calls by the fused passes
+    // (not under stream capture, where the host flag would run ahead of the
+    // captured work)
+    RET_IF(get_agrid(p, nimg));
+    const bool capturing = stream_capturing(p);
+    const bool was_zero = p->agrid_zero && !capturing;
+    const int rezero = use_fused_fft(p) ? 1 : 0;
+    p->agrid_zero = 0;
+    RET_IF(need_points(p));
+    if (!y && p->M > 0) return fail(MI355NUFFT_EINVAL, "y is required");
+    int rc = p->precision ? spread_impl<double>(p, y, p->d_agrid, nreps, was_zero)
+                          : spread_impl<float>(p, y, p->d_agrid, nreps, was_zero);
+    RET_IF(rc);
+    rc = p->precision ? adjoint_image_impl<double>(p, p->d_agrid, x, xs, nimg, rezero)
+                      : adjoint_image_impl<float>(p, p->d_agrid, x, xs, nimg, rezero);
+    RET_IF(rc);
+    if (!capturing) p->agrid_zero = rezero;
+    return MI355NUFFT_OK;
 }
 
 int mi355nufft_table1_forward(int ndim, const int64_t* Kd, const int32_t* Jd, int32_t L, int precision,

@@ -140,3 +140,62 @@ def test_set_fft_rejects_bad_method():
     with pytest.raises(KeyError):
         A.set_fft("cufft")
     assert A._lib.mi355nufft_set_fft(A._plan, 7) == 1
+
+
+def _adjoint_call(A, y, nreps, grid):
+    out = torch.empty((nreps,) + tuple(reversed(A.Nd)), dtype=A._tcdt, device=dev())
+    A._sync_stream()
+    gp = None if grid is None else ctypes.c_void_p(grid.data_ptr())
+    _lib.check(A._lib.mi355nufft_adjoint(A._plan, ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                         _lib.i64arr(_strides(out)), nreps, gp))
+    return out
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+@pytest.mark.parametrize("Nd,Kd", [((48, 40), (64, 128)), ((12, 10, 14), (16, 32, 32)), ((90,), (256,))])
+def test_plan_grid_kept_zero_between_adjoints(precision, Nd, Kd):
+    """Adjoints through the plan-owned grid (cleared by the previous adjoint's
+    fused passes) equal adjoints through a fresh caller grid (cleared by the
+    scatter): repeated calls, a smaller then a larger batch, forwards in
+    between, and a switch to rocFFT and back."""
+    om = np.random.default_rng(3).uniform(-np.pi, np.pi, (700, len(Nd)))
+    A = NufftBase(Nd, om, Jd=6, Kd=Kd, Ld=256, precision=precision, device=dev())
+    nk = int(np.prod(Kd))
+    tol = 3 * TOL[precision]
+    for step, (nreps, fft) in enumerate([(2, "auto"), (2, "auto"), (1, "auto"), (3, "auto"), (2, "rocfft"),
+                                         (2, "auto"), (2, "auto")]):
+        A.set_fft(fft)
+        y = torch.from_numpy(trj.complex_gaussian((nreps, A.M), seed=20 + step)).to(dev()).to(A._tcdt)
+        ref = _adjoint_call(A, y, nreps, torch.full((nreps, nk), 7.0, dtype=A._tcdt, device=dev()))
+        got = _adjoint_call(A, y, nreps, None)
+        assert rel_l2(got.cpu().numpy(), ref.cpu().numpy()) <= tol, (step, nreps, fft)
+        if step % 2:
+            xn, xt = _image(Nd, nreps, A._tcdt, seed=30 + step)
+            strides = _strides(xt)
+            yy = torch.empty((nreps, A.M), dtype=A._tcdt, device=dev())
+            _lib.check(A._lib.mi355nufft_forward(A._plan, ctypes.c_void_p(xt.data_ptr()), _lib.i64arr(strides),
+                                                 ctypes.c_void_p(yy.data_ptr()), nreps, None))
+
+
+def test_plan_grid_adjoint_under_graph_capture():
+    """A captured adjoint through the plan-owned grid clears it itself; replays
+    with eager adjoints in between all equal the eager result."""
+    Nd, Kd = (64, 64), (128, 128)
+    A = NufftBase(Nd, trj.golden_angle_2d(40, 128), Jd=6, Kd=Kd, Ld=1024, device=dev())
+    y = torch.from_numpy(trj.complex_gaussian((1, A.M), seed=40)).to(dev()).to(A._tcdt)
+    ref = A._adjoint_dev(y, 1).clone()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        A._adjoint_dev(y, 1)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        out = A._adjoint_dev(y, 1)
+    for _ in range(3):
+        g.replay()
+        torch.cuda.synchronize()
+        assert rel_l2(out.cpu().numpy(), ref.cpu().numpy()) <= 1e-6
+        e = A._adjoint_dev(y, 1)
+        assert rel_l2(e.cpu().numpy(), ref.cpu().numpy()) <= 1e-6
