@@ -40,6 +40,7 @@ struct BinState {
     int* perm;       // (M) sorted position -> point index
     int* sub_bin;    // (max_sub) bin of each subproblem
     int* sub_start;  // (max_sub) first sorted position of each subproblem
+    int* sub_end;    // (max_sub) one past its last sorted position
     int* n_sub;      // (1) number of subproblems
     int* bin_sub;    // (nbins) first subproblem of each bin
     int* bkt;        // (nbins * kBanks) points per (bin, bank key)
@@ -145,6 +146,7 @@ __global__ __launch_bounds__(1024) void bin_scan_kernel(BinState bs, int nbins, 
         for (int i = 0; i < ns && run_s + i < max_sub; ++i) {
             bs.sub_bin[run_s + i] = b;
             bs.sub_start[run_s + i] = run_p + i * q + min(i, rem);
+            bs.sub_end[run_s + i] = run_p + (i + 1) * q + min(i + 1, rem);
         }
         run_p += c;
         run_s += ns;
@@ -227,13 +229,7 @@ struct Sub {
 __device__ __forceinline__ void load_sub(const BinGeom& bg, const BinState& bs, int ndim, int sid, Sub& s) {
     const int bin = bs.sub_bin[sid];
     s.start = bs.sub_start[sid];
-    {  // subproblem j of its bin holds q + (j < rem) points (bin_scan_kernel)
-        const int c = bs.counts[bin];
-        const int ns = (c + bg.P - 1) / bg.P;
-        const int q = c / ns, rem = c - q * ns;
-        const int j = sid - bs.bin_sub[bin];
-        s.end = s.start + q + (j < rem ? 1 : 0);
-    }
+    s.end = bs.sub_end[sid];
     s.frame = bin / bg.nbins_frame;
     int q = bin - s.frame * bg.nbins_frame;
     for (int d = 0; d < 3; ++d) {
@@ -299,6 +295,140 @@ __device__ __forceinline__ void stage_tile(const TiledArgs<T, CK>& a, const Sub&
     }
 }
 
+// Coefficients and tile offsets of one point's J^d taps (sorted position i).
+template <typename T, bool CK, int JMAX>
+struct PointTaps {
+    T cr[3][JMAX];
+    T ci[3][JMAX];   // complex kernel only
+    int l[3];        // first tap in tile coordinates
+    int64_t m;       // point index in the caller's order
+    cx<T> ph;        // per-sample phase (1 if none)
+};
+
+template <typename T, int NDIM, bool CK, int JMAX, bool FIXED>
+__device__ __forceinline__ void point_taps(const TiledArgs<T, CK>& a, const AxisGeom* ax, const Sub& s, int i,
+                                           PointTaps<T, CK, JMAX>& pt) {
+    const InterpArgs<T, CK>& ia = a.ia;
+    pt.m = a.bs.perm[i];
+    pt.ph = a.phs ? a.phs[i] : mkcx<T>((T)1, (T)0);
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) {
+        const T t = a.tms[(int64_t)d * ia.M + i];
+        int k0;
+        if constexpr (CK) k0 = axis_coefs<T, JMAX, FIXED>(t, ia.L, ax[d], ia.h[d], pt.cr[d], pt.ci[d]);
+        else k0 = axis_coefs<T, JMAX, FIXED>(t, ia.L, ax[d], ia.h[d], pt.cr[d]);
+        pt.l[d] = k0 - s.o[d];
+    }
+}
+
+// sum over the J^d taps of the staged tile, in the reference's order
+template <typename T, int NDIM, bool CK, int JMAX, bool FIXED>
+__device__ __forceinline__ cx<T> gather_taps(const cx<T>* tile, const PointTaps<T, CK, JMAX>& pt,
+                                             const AxisGeom* ax, int T0, int T1, bool has_ph) {
+    T sr = 0, si = 0;
+    if constexpr (NDIM == 1) {
+#pragma unroll
+        for (int j1 = 0; j1 < JMAX; ++j1)
+            if (FIXED || j1 < ax[0].J) {
+                const cx<T> v = tile[pt.l[0] + j1];
+                madd<T, CK>(sr, si, pt.cr[0][j1], pt.ci[0][j1], v.x, v.y);
+            }
+    } else if constexpr (NDIM == 2) {
+#pragma unroll
+        for (int j2 = 0; j2 < JMAX; ++j2)
+            if (FIXED || j2 < ax[1].J) {
+                const cx<T>* row = tile + (pt.l[1] + j2) * T0 + pt.l[0];
+                T s1r = 0, s1i = 0;
+#pragma unroll
+                for (int j1 = 0; j1 < JMAX; ++j1)
+                    if (FIXED || j1 < ax[0].J) {
+                        const cx<T> v = row[j1];
+                        madd<T, CK>(s1r, s1i, pt.cr[0][j1], pt.ci[0][j1], v.x, v.y);
+                    }
+                madd<T, CK>(sr, si, pt.cr[1][j2], pt.ci[1][j2], s1r, s1i);
+            }
+    } else {
+#pragma unroll(FIXED ? JMAX : 1)
+        for (int j3 = 0; j3 < JMAX; ++j3)
+            if (FIXED || j3 < ax[2].J) {
+                T s2r = 0, s2i = 0;
+#pragma unroll
+                for (int j2 = 0; j2 < JMAX; ++j2)
+                    if (FIXED || j2 < ax[1].J) {
+                        const cx<T>* row = tile + ((pt.l[2] + j3) * T1 + (pt.l[1] + j2)) * T0 + pt.l[0];
+                        T s1r = 0, s1i = 0;
+#pragma unroll
+                        for (int j1 = 0; j1 < JMAX; ++j1)
+                            if (FIXED || j1 < ax[0].J) {
+                                const cx<T> v = row[j1];
+                                madd<T, CK>(s1r, s1i, pt.cr[0][j1], pt.ci[0][j1], v.x, v.y);
+                            }
+                        madd<T, CK>(s2r, s2i, pt.cr[1][j2], pt.ci[1][j2], s1r, s1i);
+                    }
+                madd<T, CK>(sr, si, pt.cr[2][j3], pt.ci[2][j3], s2r, s2i);
+            }
+    }
+    if (!has_ph) return mkcx<T>(sr, si);
+    cx<T> o;
+    o.x = sr * pt.ph.x - si * pt.ph.y;
+    o.y = sr * pt.ph.y + si * pt.ph.x;
+    return o;
+}
+
+// add conj(coefficient product) * f at every tap of the float64 LDS tile
+template <typename T, int NDIM, bool CK, int JMAX, bool FIXED>
+__device__ __forceinline__ void scatter_taps(double* tre, double* tim, const PointTaps<T, CK, JMAX>& pt, cx<T> f,
+                                             const AxisGeom* ax, int T0, int T1) {
+    if constexpr (NDIM == 1) {
+#pragma unroll
+        for (int j1 = 0; j1 < JMAX; ++j1)
+            if (FIXED || j1 < ax[0].J) {
+                T vr, vi;
+                cmul_conj<T, CK>(vr, vi, pt.cr[0][j1], pt.ci[0][j1], f.x, f.y);
+                atomicAdd(tre + pt.l[0] + j1, (double)vr);
+                atomicAdd(tim + pt.l[0] + j1, (double)vi);
+            }
+    } else if constexpr (NDIM == 2) {
+#pragma unroll
+        for (int j2 = 0; j2 < JMAX; ++j2)
+            if (FIXED || j2 < ax[1].J) {
+                T v2r, v2i;
+                cmul_conj<T, CK>(v2r, v2i, pt.cr[1][j2], pt.ci[1][j2], f.x, f.y);
+                const int row = (pt.l[1] + j2) * T0 + pt.l[0];
+#pragma unroll
+                for (int j1 = 0; j1 < JMAX; ++j1)
+                    if (FIXED || j1 < ax[0].J) {
+                        T vr, vi;
+                        cmul_conj<T, CK>(vr, vi, pt.cr[0][j1], pt.ci[0][j1], v2r, v2i);
+                        atomicAdd(tre + row + j1, (double)vr);
+                        atomicAdd(tim + row + j1, (double)vi);
+                    }
+            }
+    } else {
+#pragma unroll(FIXED ? JMAX : 1)
+        for (int j3 = 0; j3 < JMAX; ++j3)
+            if (FIXED || j3 < ax[2].J) {
+                T v3r, v3i;
+                cmul_conj<T, CK>(v3r, v3i, pt.cr[2][j3], pt.ci[2][j3], f.x, f.y);
+#pragma unroll
+                for (int j2 = 0; j2 < JMAX; ++j2)
+                    if (FIXED || j2 < ax[1].J) {
+                        T v2r, v2i;
+                        cmul_conj<T, CK>(v2r, v2i, pt.cr[1][j2], pt.ci[1][j2], v3r, v3i);
+                        const int row = ((pt.l[2] + j3) * T1 + (pt.l[1] + j2)) * T0 + pt.l[0];
+#pragma unroll
+                        for (int j1 = 0; j1 < JMAX; ++j1)
+                            if (FIXED || j1 < ax[0].J) {
+                                T vr, vi;
+                                cmul_conj<T, CK>(vr, vi, pt.cr[0][j1], pt.ci[0][j1], v2r, v2i);
+                                atomicAdd(tre + row + j1, (double)vr);
+                                atomicAdd(tim + row + j1, (double)vi);
+                            }
+                    }
+            }
+    }
+}
+
 template <typename T, int NDIM, bool CK, int JMAX, bool FIXED>
 __global__ __launch_bounds__(256) void interp_tiled_kernel(TiledArgs<T, CK> a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -310,80 +440,27 @@ __global__ __launch_bounds__(256) void interp_tiled_kernel(TiledArgs<T, CK> a) {
     const int tid = threadIdx.x;
     const int n_sub = a.bs.n_sub[0];
     for (int sid = blockIdx.x; sid < n_sub; sid += gridDim.x) {
-    Sub s;
-    load_sub(bg, a.bs, NDIM, sid, s);
-    for (int r = 0; r < ia.nreps; ++r) {
-        stage_tile<T, NDIM, CK>(a, s, ax, ia.grid + ((int64_t)s.frame * ia.nreps + r) * ia.kstride, tile);
-        __syncthreads();
-        for (int i = s.start + tid; i < s.end; i += blockDim.x) {
-            const int64_t m = a.bs.perm[i];
-            T cr[3][JMAX], ci[3][JMAX];
-            int l[3];
-#pragma unroll
-            for (int d = 0; d < NDIM; ++d) {
-                const T t = a.tms[(int64_t)d * ia.M + i];
-                int k0;
-                if constexpr (CK) k0 = axis_coefs<T, JMAX, FIXED>(t, ia.L, ax[d], ia.h[d], cr[d], ci[d]);
-                else k0 = axis_coefs<T, JMAX, FIXED>(t, ia.L, ax[d], ia.h[d], cr[d]);
-                l[d] = k0 - s.o[d];
+        Sub s;
+        load_sub(bg, a.bs, NDIM, sid, s);
+        for (int r = 0; r < ia.nreps; ++r) {
+            stage_tile<T, NDIM, CK>(a, s, ax, ia.grid + ((int64_t)s.frame * ia.nreps + r) * ia.kstride, tile);
+            __syncthreads();
+            for (int i = s.start + tid; i < s.end; i += blockDim.x) {
+                PointTaps<T, CK, JMAX> pt;
+                point_taps<T, NDIM, CK, JMAX, FIXED>(a, ax, s, i, pt);
+                ia.out[(int64_t)r * ia.M + pt.m] = gather_taps<T, NDIM, CK, JMAX, FIXED>(tile, pt, ax, T0, T1, a.phs);
             }
-            T sr = 0, si = 0;
-            if constexpr (NDIM == 1) {
-#pragma unroll
-                for (int j1 = 0; j1 < JMAX; ++j1)
-                    if (FIXED || j1 < ax[0].J) {
-                        const cx<T> v = tile[l[0] + j1];
-                        madd<T, CK>(sr, si, cr[0][j1], ci[0][j1], v.x, v.y);
-                    }
-            } else if constexpr (NDIM == 2) {
-#pragma unroll
-                for (int j2 = 0; j2 < JMAX; ++j2)
-                    if (FIXED || j2 < ax[1].J) {
-                        const cx<T>* row = tile + (l[1] + j2) * T0 + l[0];
-                        T s1r = 0, s1i = 0;
-#pragma unroll
-                        for (int j1 = 0; j1 < JMAX; ++j1)
-                            if (FIXED || j1 < ax[0].J) {
-                                const cx<T> v = row[j1];
-                                madd<T, CK>(s1r, s1i, cr[0][j1], ci[0][j1], v.x, v.y);
-                            }
-                        madd<T, CK>(sr, si, cr[1][j2], ci[1][j2], s1r, s1i);
-                    }
-            } else {
-#pragma unroll(FIXED ? JMAX : 1)
-                for (int j3 = 0; j3 < JMAX; ++j3)
-                    if (FIXED || j3 < ax[2].J) {
-                        T s2r = 0, s2i = 0;
-#pragma unroll
-                        for (int j2 = 0; j2 < JMAX; ++j2)
-                            if (FIXED || j2 < ax[1].J) {
-                                const cx<T>* row = tile + ((l[2] + j3) * T1 + (l[1] + j2)) * T0 + l[0];
-                                T s1r = 0, s1i = 0;
-#pragma unroll
-                                for (int j1 = 0; j1 < JMAX; ++j1)
-                                    if (FIXED || j1 < ax[0].J) {
-                                        const cx<T> v = row[j1];
-                                        madd<T, CK>(s1r, s1i, cr[0][j1], ci[0][j1], v.x, v.y);
-                                    }
-                                madd<T, CK>(s2r, s2i, cr[1][j2], ci[1][j2], s1r, s1i);
-                            }
-                        madd<T, CK>(sr, si, cr[2][j3], ci[2][j3], s2r, s2i);
-                    }
-            }
-            cx<T> o;
-            if (a.phs) {
-                const cx<T> ph = a.phs[i];
-                o.x = sr * ph.x - si * ph.y;
-                o.y = sr * ph.y + si * ph.x;
-            } else {
-                o.x = sr;
-                o.y = si;
-            }
-            ia.out[(int64_t)r * ia.M + m] = o;
+            __syncthreads();
         }
-        __syncthreads();
     }
-    }
+}
+
+template <typename T, bool CK, int JMAX>
+__device__ __forceinline__ cx<T> sample_times_conj_phase(const InterpArgs<T, CK>& ia, int r,
+                                                         const PointTaps<T, CK, JMAX>& pt, bool has_ph) {
+    const cx<T> f = ia.in[(int64_t)r * ia.M + pt.m];
+    if (!has_ph) return f;
+    return mkcx<T>(f.x * pt.ph.x + f.y * pt.ph.y, f.y * pt.ph.x - f.x * pt.ph.y);
 }
 
 template <typename T, int NDIM, bool CK, int JMAX, bool FIXED>
@@ -392,9 +469,9 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
     const InterpArgs<T, CK>& ia = a.ia;
     const BinGeom& bg = a.bg;
     // The tile accumulates in float64 for both precisions: on gfx950 an LDS
-    // float32 atomic add occupies the LDS for ~8x the cycles of a float64 one
-    // (measured: SQ_LDS_IDX_ACTIVE per ds_add_f32 vs ds_add_f64), and the
-    // wider sum is also more accurate than the reference's float32 sum.
+    // float32 atomic add costs ~20x the cycles of a float64 one (measured
+    // ds_add_f32 vs ds_add_f64 throughput), and the wider sum is also more
+    // accurate than the reference's float32 sum.
     double* tre = reinterpret_cast<double*>(smem);
     double* tim = tre + bg.tile;
     const AxisGeom ax[3] = {ia.ax[0], ia.ax[1], ia.ax[2]};
@@ -402,111 +479,48 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
     const int tid = threadIdx.x;
     const int n_sub = a.bs.n_sub[0];
     for (int sid = blockIdx.x; sid < n_sub; sid += gridDim.x) {
-    Sub s;
-    load_sub(bg, a.bs, NDIM, sid, s);
-    for (int r = 0; r < ia.nreps; ++r) {
-        for (int e = tid; e < bg.tile; e += blockDim.x) {
-            tre[e] = 0.0;
-            tim[e] = 0.0;
-        }
-        __syncthreads();
-        for (int i = s.start + tid; i < s.end; i += blockDim.x) {
-            const int64_t m = a.bs.perm[i];
-            T cr[3][JMAX], ci[3][JMAX];
-            int l[3];
-#pragma unroll
-            for (int d = 0; d < NDIM; ++d) {
-                const T t = a.tms[(int64_t)d * ia.M + i];
-                int k0;
-                if constexpr (CK) k0 = axis_coefs<T, JMAX, FIXED>(t, ia.L, ax[d], ia.h[d], cr[d], ci[d]);
-                else k0 = axis_coefs<T, JMAX, FIXED>(t, ia.L, ax[d], ia.h[d], cr[d]);
-                l[d] = k0 - s.o[d];
+        Sub s;
+        load_sub(bg, a.bs, NDIM, sid, s);
+        for (int r = 0; r < ia.nreps; ++r) {
+            for (int e = tid; e < bg.tile; e += blockDim.x) {
+                tre[e] = 0.0;
+                tim[e] = 0.0;
             }
-            cx<T> f = ia.in[(int64_t)r * ia.M + m];
-            if (a.phs) {
-                const cx<T> ph = a.phs[i];
-                const T fr = f.x * ph.x + f.y * ph.y;
-                const T fi = f.y * ph.x - f.x * ph.y;
-                f.x = fr;
-                f.y = fi;
-            }
-            if constexpr (NDIM == 1) {
-#pragma unroll
-                for (int j1 = 0; j1 < JMAX; ++j1)
-                    if (FIXED || j1 < ax[0].J) {
-                        T vr, vi;
-                        cmul_conj<T, CK>(vr, vi, cr[0][j1], ci[0][j1], f.x, f.y);
-                        atomicAdd(tre + l[0] + j1, (double)vr);
-                        atomicAdd(tim + l[0] + j1, (double)vi);
-                    }
-            } else if constexpr (NDIM == 2) {
-#pragma unroll
-                for (int j2 = 0; j2 < JMAX; ++j2)
-                    if (FIXED || j2 < ax[1].J) {
-                        T v2r, v2i;
-                        cmul_conj<T, CK>(v2r, v2i, cr[1][j2], ci[1][j2], f.x, f.y);
-                        const int row = (l[1] + j2) * T0 + l[0];
-#pragma unroll
-                        for (int j1 = 0; j1 < JMAX; ++j1)
-                            if (FIXED || j1 < ax[0].J) {
-                                T vr, vi;
-                                cmul_conj<T, CK>(vr, vi, cr[0][j1], ci[0][j1], v2r, v2i);
-                                atomicAdd(tre + row + j1, (double)vr);
-                                atomicAdd(tim + row + j1, (double)vi);
-                            }
-                    }
-            } else {
-#pragma unroll(FIXED ? JMAX : 1)
-                for (int j3 = 0; j3 < JMAX; ++j3)
-                    if (FIXED || j3 < ax[2].J) {
-                        T v3r, v3i;
-                        cmul_conj<T, CK>(v3r, v3i, cr[2][j3], ci[2][j3], f.x, f.y);
-#pragma unroll
-                        for (int j2 = 0; j2 < JMAX; ++j2)
-                            if (FIXED || j2 < ax[1].J) {
-                                T v2r, v2i;
-                                cmul_conj<T, CK>(v2r, v2i, cr[1][j2], ci[1][j2], v3r, v3i);
-                                const int row = ((l[2] + j3) * T1 + (l[1] + j2)) * T0 + l[0];
-#pragma unroll
-                                for (int j1 = 0; j1 < JMAX; ++j1)
-                                    if (FIXED || j1 < ax[0].J) {
-                                        T vr, vi;
-                                        cmul_conj<T, CK>(vr, vi, cr[0][j1], ci[0][j1], v2r, v2i);
-                                        atomicAdd(tre + row + j1, (double)vr);
-                                        atomicAdd(tim + row + j1, (double)vi);
-                                    }
-                            }
-                    }
-            }
-        }
-        __syncthreads();
-        if (a.tiles) {  // two-phase: write the whole tile to its slot, no phase
-            cx<T>* __restrict__ slot = a.tiles + ((int64_t)sid * ia.nreps + r) * bg.tile;
-            for (int e = tid; e < bg.tile; e += blockDim.x) slot[e] = mkcx<T>((T)tre[e], (T)tim[e]);
             __syncthreads();
-            continue;
-        }
-        cx<T>* __restrict__ g = ia.grid_out + ((int64_t)s.frame * ia.nreps + r) * ia.kstride;
-        for (int e = tid; e < bg.tile; e += blockDim.x) {
-            const double ar = tre[e], ai = tim[e];
-            if (ar == 0.0 && ai == 0.0) continue;
-            T vr = (T)ar, vi = (T)ai;
-            int g0, g1, g2;
-            const int64_t gi = tile_to_grid(e, bg, s, ax, g0, g1, g2);
-            if (a.use_pb) {  // * conj(phase_before)
-                const T c[3] = {a.pb[0], a.pb[1], a.pb[2]};
-                const cx<T> p = cis_tile(phase_before_arg<T>(c, NDIM, g0, g1, g2));
-                const T wr = vr * p.x + vi * p.y;
-                const T wi = vi * p.x - vr * p.y;
-                vr = wr;
-                vi = wi;
+            for (int i = s.start + tid; i < s.end; i += blockDim.x) {
+                PointTaps<T, CK, JMAX> pt;
+                point_taps<T, NDIM, CK, JMAX, FIXED>(a, ax, s, i, pt);
+                scatter_taps<T, NDIM, CK, JMAX, FIXED>(tre, tim, pt, sample_times_conj_phase<T, CK, JMAX>(ia, r, pt, a.phs),
+                                                      ax, T0, T1);
             }
-            T* q = reinterpret_cast<T*>(g + gi);
-            unsafeAtomicAdd(q, vr);
-            unsafeAtomicAdd(q + 1, vi);
+            __syncthreads();
+            if (a.tiles) {  // two-phase: write the whole tile to its slot, no phase
+                cx<T>* __restrict__ slot = a.tiles + ((int64_t)sid * ia.nreps + r) * bg.tile;
+                for (int e = tid; e < bg.tile; e += blockDim.x) slot[e] = mkcx<T>((T)tre[e], (T)tim[e]);
+                __syncthreads();
+                continue;
+            }
+            cx<T>* __restrict__ g = ia.grid_out + ((int64_t)s.frame * ia.nreps + r) * ia.kstride;
+            for (int e = tid; e < bg.tile; e += blockDim.x) {
+                const double ar = tre[e], ai = tim[e];
+                if (ar == 0.0 && ai == 0.0) continue;
+                T vr = (T)ar, vi = (T)ai;
+                int g0, g1, g2;
+                const int64_t gi = tile_to_grid(e, bg, s, ax, g0, g1, g2);
+                if (a.use_pb) {  // * conj(phase_before)
+                    const T c[3] = {a.pb[0], a.pb[1], a.pb[2]};
+                    const cx<T> p = cis_tile(phase_before_arg<T>(c, NDIM, g0, g1, g2));
+                    const T wr = vr * p.x + vi * p.y;
+                    const T wi = vi * p.x - vr * p.y;
+                    vr = wr;
+                    vi = wi;
+                }
+                T* q = reinterpret_cast<T*>(g + gi);
+                unsafeAtomicAdd(q, vr);
+                unsafeAtomicAdd(q + 1, vi);
+            }
+            __syncthreads();
         }
-        __syncthreads();
-    }
     }
 }
 

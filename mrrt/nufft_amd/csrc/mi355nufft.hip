@@ -117,7 +117,7 @@ struct mi355nufft_plan {
     int binned = 0;           // bin state valid for the current points
     int* d_bin_i = nullptr;   // one allocation: bin_of, rank, perm (3*M_cap)
     int* d_bin_b = nullptr;   // counts, bin_start, bin_sub, bank-key counts ((3 + kBanks) * nbins_cap)
-    int* d_sub = nullptr;     // sub_bin, sub_start (2*sub_cap) + n_sub
+    int* d_sub = nullptr;     // sub_bin, sub_start, sub_end (3*sub_cap) + n_sub
     void* d_tms = nullptr;    // tm in sorted order (ndim * M_cap real)
     void* d_phs = nullptr;    // per-sample phase in sorted order (M_cap complex)
     int64_t sorted_cap = 0;
@@ -730,7 +730,8 @@ BinState bin_state(const mi355nufft_plan* p) {
     bs.bin_sub = bs.counts + 2 * bc;
     bs.sub_bin = p->d_sub;
     bs.sub_start = p->d_sub + sc;
-    bs.n_sub = p->d_sub + 2 * sc;
+    bs.sub_end = p->d_sub + 2 * sc;
+    bs.n_sub = p->d_sub + 3 * sc;
     bs.frame_off = p->nframes > 1 ? p->d_frame_off : nullptr;
     bs.nframes = p->nframes;
     return bs;
@@ -770,7 +771,7 @@ int bin_points_impl(mi355nufft_plan* p) {
     if ((int64_t)p->bg.nbins * kBanks >= ((int64_t)1 << 31))
         return fail(MI355NUFFT_EINVAL, "too many bins (%d) for the bin sort", p->bg.nbins);
     RET_IF(grow(p, &p->d_bin_b, p->bin_b_cap, (int64_t)p->bg.nbins, 3 + kBanks));
-    RET_IF(grow(p, &p->d_sub, p->bin_s_cap, max_sub, 2));
+    RET_IF(grow(p, &p->d_sub, p->bin_s_cap, max_sub, 3));
     BinState bs = bin_state(p);
     InterpArgs<T, false> ia;
     int32_t J[3] = {p->J[0], p->J[1], p->J[2]};
