@@ -352,6 +352,14 @@ def main():
             yh = run.A.fft(xh)
             run.A.adj(yh)
         extra["c2_host_arrays_pairs_per_s"] = round(nh / (time.perf_counter() - t0), 2)
+        # Toeplitz normal operator A^H A on the same points (circulant 512^2)
+        from mrrt.nufft_amd.toeplitz import ToeplitzNormal
+        tn = ToeplitzNormal(run.A)
+        xt = run.x
+        t_n = _time_fn(lambda: tn.apply_dev(xt), args.steps, args.warmup)
+        extra["c2_toeplitz_normal_per_s"] = round(1000.0 / t_n, 2)
+        extra["c2_toeplitz_vs_pair_speedup"] = round(ms / t_n, 3)
+        del tn
         r2 = GpuRun(c2_workload("double"))
         t2 = r2.time(r2.pair, max(5, args.steps // 2), 3)
         extra["c2_complex128_pairs_per_s"] = round(1000.0 / t2, 2)

@@ -40,6 +40,7 @@ enum {
 };
 
 typedef struct mi355nufft_plan mi355nufft_plan;
+typedef struct mi355nufft_toeplitz mi355nufft_toeplitz;
 
 /* Message for the last failing call on this thread ("" if none). */
 const char *mi355nufft_last_error(void);
@@ -304,6 +305,37 @@ int mi355nufft_table1_adjoint(int ndim, const int64_t *Kd, const int32_t *Jd,
                               const void *fm, const void *const *h,
                               const void *tm, int64_t M, void *ck,
                               int32_t nreps, void *stream);
+
+/* ---- Toeplitz normal operator ---- */
+
+/*
+ * A^H W A of a table1 NUFFT as one circulant convolution.  (A^H W A x)[n] =
+ * sum_n' x[n'] psf[n - n'] with psf[d] = c * sum_m w_m exp(i omega_m . d)
+ * (c the operator's scale: adjoint_scalefactor, / prod(Kd) with ortho).
+ * The plan embeds psf in a power-of-two grid K_d >= 2 N_d (16 <= K_d <= 2048,
+ * complex128 <= 1024) and applies
+ *   y = crop_N(IFFT_K(FFT_K(circ(psf)) * FFT_K(pad_K(x))))
+ * with the pruned FFT passes (the spectrum multiply fused into the first
+ * inverse pass).  Builds on nufft_adj(..., return_psf=True, grid_only=True)
+ * (mrrt/nufft/_nufft.py:1459-1518), where the reference stops at the
+ * gridded PSF.
+ */
+int mi355nufft_toeplitz_create(mi355nufft_toeplitz **plan, int ndim, const int64_t *Nd,
+                               int precision, void *stream);
+int mi355nufft_toeplitz_destroy(mi355nufft_toeplitz *plan);
+int mi355nufft_toeplitz_set_stream(mi355nufft_toeplitz *plan, void *stream);
+/* circulant grid sizes K_d (ndim entries) */
+int mi355nufft_toeplitz_grid(const mi355nufft_toeplitz *plan, int64_t *Kd);
+/*
+ * psf: device complex array on the lattice d_e in [-N_e, N_e), element
+ * (d_0 + N_0, d_1 + N_1, ...) at psf[sum_e (d_e + N_e) * ps[e]] (elements);
+ * the d_e = -N_e entries are not used.  This is what an adjoint NUFFT of w
+ * with image size 2 N and n_shift = N produces, scaled by c.
+ */
+int mi355nufft_toeplitz_set_psf(mi355nufft_toeplitz *plan, const void *psf, const int64_t *ps);
+/* y = (A^H W A) x for nimg images; x and y strides as mi355nufft_forward's x. */
+int mi355nufft_toeplitz_apply(mi355nufft_toeplitz *plan, const void *x, const int64_t *xs,
+                              void *y, const int64_t *ys, int32_t nimg);
 
 #ifdef __cplusplus
 }

@@ -60,6 +60,12 @@ _SIGNATURES = {
     "mi355nufft_forward_grid": (c_int, [c_vp, c_vp, c_i64p, c_vp, c_i32]),
     "mi355nufft_adjoint_image": (c_int, [c_vp, c_vp, c_vp, c_i64p, c_i32]),
     "mi355nufft_set_fft": (c_int, [c_vp, c_int]),
+    "mi355nufft_toeplitz_create": (c_int, [ctypes.POINTER(c_vp), c_int, c_i64p, c_int, c_vp]),
+    "mi355nufft_toeplitz_destroy": (c_int, [c_vp]),
+    "mi355nufft_toeplitz_set_stream": (c_int, [c_vp, c_vp]),
+    "mi355nufft_toeplitz_grid": (c_int, [c_vp, c_i64p]),
+    "mi355nufft_toeplitz_set_psf": (c_int, [c_vp, c_vp, c_i64p]),
+    "mi355nufft_toeplitz_apply": (c_int, [c_vp, c_vp, c_i64p, c_vp, c_i64p, c_i32]),
     "mi355nufft_table1_forward": (c_int, [c_int, c_i64p, c_i32p, c_i32, c_int, c_int, c_vp, c_vpp, c_vp, c_i64,
                                           c_vp, c_i32, c_vp]),
     "mi355nufft_table1_adjoint": (c_int, [c_int, c_i64p, c_i32p, c_i32, c_int, c_int, c_vp, c_vpp, c_vp, c_i64,

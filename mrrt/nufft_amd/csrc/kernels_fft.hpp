@@ -173,6 +173,7 @@ struct FftRowGeom {
     int64_t xs[4];     // image strides: axes 0..2, repetition
     int64_t nlines;    // N1 * N2 * nimg
     int lpw;           // lines per workgroup (blockDim = lpw * TL)
+    const void* mult;  // adjoint: complex T grid (one image) multiplying the loaded rows, or null
 };
 
 template <typename T, int K>
@@ -212,7 +213,7 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_rows_pad_kernel(const cx<T
     for (int t = tl; t < K; t += TL) {
         cx<T> o = mkcx<T>((T)0, (T)0);
         if (live && t < g.N[0]) {
-            const T s = deapod_factor<T>(sn, ndim, t, n2, n3);
+            const T s = sn0 ? deapod_factor<T>(sn, ndim, t, n2, n3) : (T)1;
             const cx<T> v = x[(int64_t)t * g.xs[0] + (int64_t)n2 * g.xs[1] + (int64_t)n3 * g.xs[2] + r * g.xs[3]];
             o.x = v.x * s;
             o.y = v.y * s;
@@ -249,7 +250,12 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_rows_crop_kernel(cx<T>* __
     if (live) row_of_line(line, g, n2, n3, r);
     cx<T>* ln = reinterpret_cast<cx<T>*>(smem) + li * C::KP;
     cx<T>* in = grid + r * g.nk + ((int64_t)n3 * g.K[1] + n2) * K;
-    for (int t = tl; t < K; t += TL) ln[pad(t)] = live ? in[t] : mkcx<T>((T)0, (T)0);
+    const cx<T>* mul = g.mult ? (const cx<T>*)g.mult + ((int64_t)n3 * g.K[1] + n2) * K : nullptr;
+    for (int t = tl; t < K; t += TL) {
+        cx<T> v = live ? in[t] : mkcx<T>((T)0, (T)0);
+        if (mul && live) v = cmulc<T>(v, mul[t]);
+        ln[pad(t)] = v;
+    }
     cx<T>* tws = stage_twiddles<T, K>(smem, g.lpw, tw);
     __syncthreads();
     if (rezero && live)  // each thread clears the points it loaded
@@ -258,7 +264,7 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_rows_crop_kernel(cx<T>* __
     if (live) {
         const double* sn[3] = {sn0, sn1, sn2};
         for (int t = tl; t < g.N[0]; t += TL) {
-            const T s = deapod_factor<T>(sn, ndim, t, n2, n3);
+            const T s = sn0 ? deapod_factor<T>(sn, ndim, t, n2, n3) : (T)1;
             const cx<T> v = ln[pad(t)];
             cx<T> o;
             o.x = (v.x * scale) * s;
@@ -278,6 +284,7 @@ struct FftColGeom {
     int64_t nouter;   // A * nimg
     int lpw;          // neighbouring lines per workgroup, divides S (blockDim = lpw * TL)
     int zero_tail;    // also write zeros at n_out <= t < K
+    const void* mult; // complex T grid (one image) multiplying the loaded lines, or null
 };
 
 // Axis d >= 1 in place: lpw neighbouring lines (inner index i0..i0+lpw-1) of
@@ -294,16 +301,49 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_cols_kernel(cx<T>* __restr
     const int64_t nchunk = g.S / lpw;
     const int64_t chunk = blockIdx.x % nchunk;
     const int64_t o = blockIdx.x / nchunk;
-    const int64_t base = (o % g.A) * g.st_a + (o / g.A) * g.nk + chunk * lpw;
+    const int64_t ibase = (o % g.A) * g.st_a + chunk * lpw;  // within one image
+    const int64_t base = ibase + (o / g.A) * g.nk;
     const int li = threadIdx.x % lpw, t0 = threadIdx.x / lpw;  // load / store mapping
-    for (int t = t0; t < K; t += TL)
-        lds[li * C::KP + pad(t)] = t < g.n_in ? grid[base + (int64_t)t * g.S + li] : mkcx<T>((T)0, (T)0);
+    const cx<T>* mul = (const cx<T>*)g.mult;
+    for (int t = t0; t < K; t += TL) {
+        cx<T> v = mkcx<T>((T)0, (T)0);
+        if (t < g.n_in) {
+            v = grid[base + (int64_t)t * g.S + li];
+            if (mul) v = cmulc<T>(v, mul[ibase + (int64_t)t * g.S + li]);
+        }
+        lds[li * C::KP + pad(t)] = v;
+    }
     cx<T>* tws = stage_twiddles<T, K>(smem, g.lpw, tw);
     __syncthreads();
     fft_line<T, K, TL, SIGN>(lds + (threadIdx.x / TL) * C::KP, threadIdx.x % TL, tws);
     for (int t = t0; t < g.n_out; t += TL) grid[base + (int64_t)t * g.S + li] = lds[li * C::KP + pad(t)];
     if (g.zero_tail)
         for (int t = g.n_out + t0; t < K; t += TL) grid[base + (int64_t)t * g.S + li] = mkcx<T>((T)0, (T)0);
+}
+
+// Circulant embedding of a point-spread function given on the lattice
+// d_e in [-N_e, N_e) (element q_e = d_e + N_e, strides ps): T[d mod K] =
+// psf[d] * inv_nk for d_e > -N_e on every axis.  T must be zero beforehand;
+// K_e >= 2 N_e keeps the wrapped indices distinct.
+template <typename T>
+__global__ __launch_bounds__(256) void toeplitz_embed_kernel(const cx<T>* __restrict__ psf, cx<T>* __restrict__ t,
+                                                             int ndim, int N0, int N1, int N2, int K0, int K1,
+                                                             int K2, int64_t ps0, int64_t ps1, int64_t ps2,
+                                                             int64_t count, T inv_nk) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int L0 = 2 * N0, L1 = 2 * N1;
+    const int q0 = (int)(i % L0);
+    const int64_t r = i / L0;
+    const int q1 = ndim > 1 ? (int)(r % L1) : 0;
+    const int q2 = ndim > 2 ? (int)(r / L1) : 0;
+    const int d0 = q0 - N0, d1 = ndim > 1 ? q1 - N1 : 0, d2 = ndim > 2 ? q2 - N2 : 0;
+    if (d0 == -N0 || (ndim > 1 && d1 == -N1) || (ndim > 2 && d2 == -N2)) return;
+    const int k0 = d0 < 0 ? d0 + K0 : d0;
+    const int k1 = d1 < 0 ? d1 + K1 : d1;
+    const int k2 = d2 < 0 ? d2 + K2 : d2;
+    const cx<T> v = psf[(int64_t)q0 * ps0 + (int64_t)q1 * ps1 + (int64_t)q2 * ps2];
+    t[((int64_t)k2 * K1 + k1) * K0 + k0] = mkcx<T>(v.x * inv_nk, v.y * inv_nk);
 }
 
 }  // namespace mi355

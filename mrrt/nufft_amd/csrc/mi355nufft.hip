@@ -445,32 +445,58 @@ int fft_impl(mi355nufft_plan* p, void* grid, int32_t nreps, int direction);
 // Fused pruned FFT passes (kernels_fft.hpp) for power-of-two grids.
 bool use_fused_fft(const mi355nufft_plan* p) { return p->fft_method == 0 && p->fft_fused_ok; }
 
-FftRowGeom row_geom(const mi355nufft_plan* p, const int64_t* xs, int32_t nimg) {
+// What the fused FFT passes need of a plan: sizes, twiddles, deapodization
+// factors (null: none) and the stream.
+struct FftDev {
+    int ndim;
+    int64_t N[3], K[3], nk;
+    void* const* tw;
+    const double* const* sn;
+    hipStream_t stream;
+};
+
+FftDev fft_dev(const mi355nufft_plan* p) {
+    FftDev f;
+    f.ndim = p->ndim;
+    for (int d = 0; d < 3; ++d) {
+        f.N[d] = p->N[d];
+        f.K[d] = p->K[d];
+    }
+    f.nk = p->nk;
+    f.tw = p->d_tw;
+    f.sn = p->d_sn;
+    f.stream = p->stream;
+    return f;
+}
+
+FftRowGeom row_geom(const FftDev& f, const int64_t* xs, int32_t nimg) {
     FftRowGeom g;
     for (int d = 0; d < 3; ++d) {
-        g.N[d] = (int)p->N[d];
-        g.K[d] = (int)p->K[d];
+        g.N[d] = (int)f.N[d];
+        g.K[d] = (int)f.K[d];
     }
-    g.nk = p->nk;
+    g.nk = f.nk;
     for (int d = 0; d < 4; ++d) g.xs[d] = 0;
-    for (int d = 0; d < p->ndim; ++d) g.xs[d] = xs[d];
-    g.xs[3] = xs[p->ndim];
+    for (int d = 0; d < f.ndim; ++d) g.xs[d] = xs[d];
+    g.xs[3] = xs[f.ndim];
     g.nlines = (int64_t)g.N[1] * g.N[2] * nimg;
+    g.mult = nullptr;
     return g;
 }
 
 // lines of axis d >= 1: LPW neighbouring lines of prod(K_0..K_{d-1}) per
 // workgroup, for every active index of the next axis and every image
-FftColGeom col_geom(const mi355nufft_plan* p, int d, bool adjoint, int32_t nimg) {
+FftColGeom col_geom(const FftDev& f, int d, bool adjoint, int32_t nimg) {
     FftColGeom g;
     g.S = 1;
-    for (int e = 0; e < d; ++e) g.S *= p->K[e];
-    g.n_in = (int)(adjoint ? p->K[d] : p->N[d]);
-    g.n_out = (int)(adjoint ? p->N[d] : p->K[d]);
-    g.A = d + 1 < p->ndim ? (int)p->N[d + 1] : 1;
-    g.st_a = g.S * p->K[d];
-    g.nk = p->nk;
+    for (int e = 0; e < d; ++e) g.S *= f.K[e];
+    g.n_in = (int)(adjoint ? f.K[d] : f.N[d]);
+    g.n_out = (int)(adjoint ? f.N[d] : f.K[d]);
+    g.A = d + 1 < f.ndim ? (int)f.N[d + 1] : 1;
+    g.st_a = g.S * f.K[d];
+    g.nk = f.nk;
     g.nouter = (int64_t)g.A * nimg;
+    g.mult = nullptr;
     return g;
 }
 
@@ -478,11 +504,19 @@ FftColGeom col_geom(const mi355nufft_plan* p, int d, bool adjoint, int32_t nimg)
 // while the pass keeps at least this many workgroups.
 constexpr int64_t kFftTargetBlocks = 128;
 
+// Options of one pass: rezero (adjoint: leave the grid it reads all zero),
+// mult (adjoint: multiply the loaded spectrum by this grid-shaped array).
+struct FftPassOpts {
+    int rezero = 0;
+    const void* mult = nullptr;
+};
+
 template <typename T, int K>
-int launch_fft_rows(mi355nufft_plan* p, bool adjoint, const void* x, void* xo, const int64_t* xs, void* grid,
-                    int32_t nimg, T scale, int rezero) {
+int launch_fft_rows(const FftDev& f, bool adjoint, const void* x, void* xo, const int64_t* xs, void* grid,
+                    int32_t nimg, T scale, FftPassOpts o) {
     using C = FftCfg<T, K>;
-    FftRowGeom g = row_geom(p, xs, nimg);
+    FftRowGeom g = row_geom(f, xs, nimg);
+    g.mult = o.mult;
     int lpw = 1;
     while (2 * lpw * C::TL <= 256 && C::lds_bytes(2 * lpw) <= (size_t)kFftLds &&
            g.nlines / (2 * lpw) >= kFftTargetBlocks)
@@ -490,25 +524,26 @@ int launch_fft_rows(mi355nufft_plan* p, bool adjoint, const void* x, void* xo, c
     g.lpw = lpw;
     const int64_t nb = (g.nlines + lpw - 1) / lpw;
     const size_t lds = C::lds_bytes(lpw);
-    const double* s1 = p->ndim > 1 ? p->d_sn[1] : p->d_sn[0];
-    const double* s2 = p->ndim > 2 ? p->d_sn[2] : p->d_sn[0];
+    const double* s0 = f.sn ? f.sn[0] : nullptr;
+    const double* s1 = f.sn && f.ndim > 1 ? f.sn[1] : s0;
+    const double* s2 = f.sn && f.ndim > 2 ? f.sn[2] : s0;
     if (adjoint)
-        hipLaunchKernelGGL((fft_rows_crop_kernel<T, K, 1>), dim3((unsigned)nb), dim3(lpw * C::TL), lds, p->stream,
-                           (cx<T>*)grid, (cx<T>*)xo, g, p->d_sn[0], s1, s2, p->ndim, scale,
-                           (const cx<T>*)p->d_tw[0], rezero);
+        hipLaunchKernelGGL((fft_rows_crop_kernel<T, K, 1>), dim3((unsigned)nb), dim3(lpw * C::TL), lds, f.stream,
+                           (cx<T>*)grid, (cx<T>*)xo, g, s0, s1, s2, f.ndim, scale, (const cx<T>*)f.tw[0],
+                           o.rezero);
     else
-        hipLaunchKernelGGL((fft_rows_pad_kernel<T, K, -1>), dim3((unsigned)nb), dim3(lpw * C::TL), lds, p->stream,
-                           (const cx<T>*)x, (cx<T>*)grid, g, p->d_sn[0], s1, s2, p->ndim,
-                           (const cx<T>*)p->d_tw[0]);
+        hipLaunchKernelGGL((fft_rows_pad_kernel<T, K, -1>), dim3((unsigned)nb), dim3(lpw * C::TL), lds, f.stream,
+                           (const cx<T>*)x, (cx<T>*)grid, g, s0, s1, s2, f.ndim, (const cx<T>*)f.tw[0]);
     LAUNCH_CHECK();
     return MI355NUFFT_OK;
 }
 
 template <typename T, int K>
-int launch_fft_cols(mi355nufft_plan* p, bool adjoint, int d, void* grid, int32_t nimg, int rezero) {
+int launch_fft_cols(const FftDev& f, bool adjoint, int d, void* grid, int32_t nimg, FftPassOpts o) {
     using C = FftCfg<T, K>;
-    FftColGeom g = col_geom(p, d, adjoint, nimg);
-    g.zero_tail = rezero;
+    FftColGeom g = col_geom(f, d, adjoint, nimg);
+    g.zero_tail = o.rezero;
+    g.mult = o.mult;
     // up to 128 contiguous bytes per row, while the pass keeps its workgroups
     int lpw = 1;
     while (2 * lpw * C::CXS <= 128 && g.S % (2 * lpw) == 0 && 2 * lpw * C::TL <= kFftMaxThreads &&
@@ -519,11 +554,11 @@ int launch_fft_cols(mi355nufft_plan* p, bool adjoint, int d, void* grid, int32_t
     if (nb >= ((int64_t)1 << 32)) return fail(MI355NUFFT_EINVAL, "FFT launch too large");
     const size_t lds = C::lds_bytes(lpw);
     if (adjoint)
-        hipLaunchKernelGGL((fft_cols_kernel<T, K, 1>), dim3((unsigned)nb), dim3(lpw * C::TL), lds, p->stream,
-                           (cx<T>*)grid, g, (const cx<T>*)p->d_tw[d]);
+        hipLaunchKernelGGL((fft_cols_kernel<T, K, 1>), dim3((unsigned)nb), dim3(lpw * C::TL), lds, f.stream,
+                           (cx<T>*)grid, g, (const cx<T>*)f.tw[d]);
     else
-        hipLaunchKernelGGL((fft_cols_kernel<T, K, -1>), dim3((unsigned)nb), dim3(lpw * C::TL), lds, p->stream,
-                           (cx<T>*)grid, g, (const cx<T>*)p->d_tw[d]);
+        hipLaunchKernelGGL((fft_cols_kernel<T, K, -1>), dim3((unsigned)nb), dim3(lpw * C::TL), lds, f.stream,
+                           (cx<T>*)grid, g, (const cx<T>*)f.tw[d]);
     LAUNCH_CHECK();
     return MI355NUFFT_OK;
 }
@@ -542,18 +577,55 @@ int launch_fft_cols(mi355nufft_plan* p, bool adjoint, int d, void* grid, int32_t
     }
 
 template <typename T>
-int fft_rows(mi355nufft_plan* p, bool adjoint, const void* x, void* xo, const int64_t* xs, void* grid,
-             int32_t nimg, T scale, int rezero) {
-#define MI355_ROWS(KK) launch_fft_rows<T, KK>(p, adjoint, x, xo, xs, grid, nimg, scale, rezero)
-    MI355_FFT_SWITCH((int)p->K[0], MI355_ROWS)
+int fft_rows(const FftDev& f, bool adjoint, const void* x, void* xo, const int64_t* xs, void* grid, int32_t nimg,
+             T scale, FftPassOpts o = FftPassOpts()) {
+#define MI355_ROWS(KK) launch_fft_rows<T, KK>(f, adjoint, x, xo, xs, grid, nimg, scale, o)
+    MI355_FFT_SWITCH((int)f.K[0], MI355_ROWS)
 #undef MI355_ROWS
 }
 
 template <typename T>
-int fft_cols(mi355nufft_plan* p, bool adjoint, int d, void* grid, int32_t nimg, int rezero) {
-#define MI355_COLS(KK) launch_fft_cols<T, KK>(p, adjoint, d, grid, nimg, rezero)
-    MI355_FFT_SWITCH((int)p->K[d], MI355_COLS)
+int fft_cols(const FftDev& f, bool adjoint, int d, void* grid, int32_t nimg, FftPassOpts o = FftPassOpts()) {
+#define MI355_COLS(KK) launch_fft_cols<T, KK>(f, adjoint, d, grid, nimg, o)
+    MI355_FFT_SWITCH((int)f.K[d], MI355_COLS)
 #undef MI355_COLS
+}
+
+// Every K_d a power of two the fused passes support at this precision: one
+// padded line and the twiddle table must fit the LDS budget (complex128
+// lines of 2048 points do not).
+bool fused_fft_sizes_ok(int ndim, const int64_t* Kd, size_t cx_size) {
+    for (int d = 0; d < ndim; ++d) {
+        const int64_t k = Kd[d];
+        if (k < kFftMinK || k > kFftMaxK || (k & (k - 1)) != 0 ||
+            (size_t)(2 * k + k / 32 + 1) * cx_size > (size_t)kFftLds)  // FftCfg::lds_bytes(1)
+            return false;
+    }
+    return true;
+}
+
+// twiddles exp(-2 pi i m / K_d), m < K_d, rounded once from double
+int make_twiddles(int ndim, const int64_t* Kd, int precision, void** tw) {
+    for (int d = 0; d < ndim; ++d) {
+        const int64_t k = Kd[d];
+        std::vector<double> w((size_t)(2 * k));
+        for (int64_t m = 0; m < k; ++m) {
+            const double a = 2.0 * M_PI * (double)m / (double)k;
+            w[(size_t)(2 * m)] = std::cos(a);
+            w[(size_t)(2 * m + 1)] = -std::sin(a);
+        }
+        const size_t bytes = (size_t)k * 2 * (precision ? sizeof(double) : sizeof(float));
+        std::vector<float> wf;
+        const void* src = w.data();
+        if (!precision) {
+            wf.assign(w.begin(), w.end());
+            src = wf.data();
+        }
+        if (hipMalloc(&tw[d], bytes) != hipSuccess) return fail(MI355NUFFT_ENOMEM, "FFT twiddle table");
+        if (hipMemcpy(tw[d], src, bytes, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(MI355NUFFT_EHIP, "FFT twiddle upload");
+    }
+    return MI355NUFFT_OK;
 }
 
 double adjoint_image_scale(const mi355nufft_plan* p) {
@@ -568,8 +640,9 @@ int forward_grid_impl(mi355nufft_plan* p, const void* x, const int64_t* xs, void
         RET_IF(deapod_pad_impl<T>(p, x, xs, grid, nimg));
         return fft_impl(p, grid, nimg, -1);
     }
-    RET_IF(fft_rows<T>(p, false, x, nullptr, xs, grid, nimg, (T)1, 0));
-    for (int d = 1; d < p->ndim; ++d) RET_IF(fft_cols<T>(p, false, d, grid, nimg, 0));
+    const FftDev f = fft_dev(p);
+    RET_IF(fft_rows<T>(f, false, x, nullptr, xs, grid, nimg, (T)1));
+    for (int d = 1; d < p->ndim; ++d) RET_IF(fft_cols<T>(f, false, d, grid, nimg));
     return MI355NUFFT_OK;
 }
 
@@ -583,8 +656,11 @@ int adjoint_image_impl(mi355nufft_plan* p, void* grid, void* x, const int64_t* x
         RET_IF(fft_impl(p, grid, nimg, 1));
         return crop_deapod_impl<T>(p, grid, x, xs, nimg);
     }
-    for (int d = p->ndim - 1; d >= 1; --d) RET_IF(fft_cols<T>(p, true, d, grid, nimg, rezero));
-    return fft_rows<T>(p, true, nullptr, x, xs, grid, nimg, (T)adjoint_image_scale(p), rezero);
+    const FftDev f = fft_dev(p);
+    FftPassOpts o;
+    o.rezero = rezero;
+    for (int d = p->ndim - 1; d >= 1; --d) RET_IF(fft_cols<T>(f, true, d, grid, nimg, o));
+    return fft_rows<T>(f, true, nullptr, x, xs, grid, nimg, (T)adjoint_image_scale(p), o);
 }
 
 template <typename T>
@@ -1034,6 +1110,97 @@ int spread_two_phase(mi355nufft_plan* p, const void* y, void* grid, int32_t nrep
     return MI355NUFFT_OK;
 }
 
+}  // namespace
+
+// Toeplitz normal operator: A^H W A of a table1 NUFFT applied as one
+// circulant convolution on a power-of-two grid K_d >= 2 N_d:
+//   y = crop_N(IFFT_K(T * FFT_K(pad_K(x)))),  T = FFT_K(circ(psf)) / prod(K).
+struct mi355nufft_toeplitz {
+    int ndim = 0;
+    int precision = 0;
+    int64_t N[3] = {1, 1, 1};
+    int64_t K[3] = {1, 1, 1};
+    int64_t nk = 1;
+    hipStream_t stream = nullptr;
+    void* d_tw[3] = {nullptr, nullptr, nullptr};
+    void* d_T = nullptr;      // spectrum of the embedded kernel, nk complex
+    int have_T = 0;
+    void* d_grid = nullptr;   // scratch, nimg * nk complex
+    size_t grid_bytes = 0;
+    size_t cx_size() const { return precision ? 2 * sizeof(double) : 2 * sizeof(float); }
+};
+
+namespace {
+
+FftDev toeplitz_dev(const mi355nufft_toeplitz* tp, bool full) {
+    FftDev f;
+    f.ndim = tp->ndim;
+    for (int d = 0; d < 3; ++d) {
+        f.N[d] = full ? tp->K[d] : tp->N[d];
+        f.K[d] = tp->K[d];
+    }
+    f.nk = tp->nk;
+    f.tw = tp->d_tw;
+    f.sn = nullptr;
+    f.stream = tp->stream;
+    return f;
+}
+
+int toeplitz_scratch(mi355nufft_toeplitz* tp, int32_t nimg) {
+    const size_t need = (size_t)nimg * (size_t)tp->nk * tp->cx_size();
+    if (need > tp->grid_bytes) {
+        if (tp->d_grid) {
+            HIP_TRY(hipStreamSynchronize(tp->stream));
+            HIP_TRY(hipFree(tp->d_grid));
+            tp->d_grid = nullptr;
+            tp->grid_bytes = 0;
+        }
+        if (hipMalloc(&tp->d_grid, need) != hipSuccess)
+            return fail(MI355NUFFT_ENOMEM, "Toeplitz grid of %zu bytes", need);
+        tp->grid_bytes = need;
+    }
+    return MI355NUFFT_OK;
+}
+
+template <typename T>
+int toeplitz_set_psf_impl(mi355nufft_toeplitz* tp, const void* psf, const int64_t* ps) {
+    // embed into the scratch grid, then FFT it into d_T
+    RET_IF(toeplitz_scratch(tp, 1));
+    RET_IF(zero_grid<T>(tp->d_grid, tp->nk, tp->stream));
+    int64_t count = 1;
+    for (int d = 0; d < tp->ndim; ++d) count *= 2 * tp->N[d];
+    const int64_t p0 = ps[0], p1 = tp->ndim > 1 ? ps[1] : 0, p2 = tp->ndim > 2 ? ps[2] : 0;
+    hipLaunchKernelGGL((toeplitz_embed_kernel<T>), dim3(nblocks(count)), dim3(kBlock), 0, tp->stream,
+                       (const cx<T>*)psf, (cx<T>*)tp->d_grid, tp->ndim, (int)tp->N[0], (int)tp->N[1], (int)tp->N[2],
+                       (int)tp->K[0], (int)tp->K[1], (int)tp->K[2], p0, p1, p2, count, (T)(1.0 / (double)tp->nk));
+    LAUNCH_CHECK();
+    // full forward FFT of the embedded kernel
+    const FftDev f = toeplitz_dev(tp, true);
+    int64_t xs[4] = {1, tp->K[0], tp->K[0] * tp->K[1], tp->nk};
+    if (tp->ndim == 1) xs[1] = tp->nk;
+    if (tp->ndim == 2) xs[2] = tp->nk;
+    RET_IF(fft_rows<T>(f, false, tp->d_grid, nullptr, xs, tp->d_T, 1, (T)1));
+    for (int d = 1; d < tp->ndim; ++d) RET_IF(fft_cols<T>(f, false, d, tp->d_T, 1));
+    tp->have_T = 1;
+    return MI355NUFFT_OK;
+}
+
+template <typename T>
+int toeplitz_apply_impl(mi355nufft_toeplitz* tp, const void* x, const int64_t* xs, void* y, const int64_t* ys,
+                        int32_t nimg) {
+    RET_IF(toeplitz_scratch(tp, nimg));
+    const FftDev f = toeplitz_dev(tp, false);
+    RET_IF(fft_rows<T>(f, false, x, nullptr, xs, tp->d_grid, nimg, (T)1));
+    for (int d = 1; d < tp->ndim; ++d) RET_IF(fft_cols<T>(f, false, d, tp->d_grid, nimg));
+    FftPassOpts o;
+    o.mult = tp->d_T;  // on the first inverse pass, which reads the whole spectrum
+    for (int d = tp->ndim - 1; d >= 1; --d) {
+        RET_IF(fft_cols<T>(f, true, d, tp->d_grid, nimg, o));
+        o.mult = nullptr;
+    }
+    return fft_rows<T>(f, true, nullptr, y, ys, tp->d_grid, nimg, (T)1, o);
+}
+
 int need_points(const mi355nufft_plan* p) {
     if (!p->d_tm && p->M > 0) return fail(MI355NUFFT_ESTATE, "points not set");
     if (p->M == 0 && !p->d_tm) return fail(MI355NUFFT_ESTATE, "mi355nufft_set_points was not called");
@@ -1101,36 +1268,12 @@ int mi355nufft_plan_create(mi355nufft_plan** plan, int ndim, const int64_t* Nd, 
             return fail(MI355NUFFT_EHIP, "table / sn upload");
         }
     }
-    // one padded line and the twiddle table must fit the LDS budget
-    // (complex128 lines of 2048 points do not)
-    p->fft_fused_ok = 1;
-    for (int d = 0; d < ndim; ++d) {
-        const int64_t k = Kd[d];
-        if (k < kFftMinK || k > kFftMaxK || (k & (k - 1)) != 0 ||
-            (size_t)(2 * k + k / 32 + 1) * p->cx_size() > (size_t)kFftLds)  // FftCfg::lds_bytes(1)
-            p->fft_fused_ok = 0;
-    }
+    p->fft_fused_ok = fused_fft_sizes_ok(ndim, Kd, p->cx_size()) ? 1 : 0;
     if (p->fft_fused_ok) {
-        for (int d = 0; d < ndim; ++d) {
-            const int64_t k = Kd[d];
-            std::vector<double> w((size_t)(2 * k));
-            for (int64_t m = 0; m < k; ++m) {
-                const double a = 2.0 * M_PI * (double)m / (double)k;
-                w[(size_t)(2 * m)] = std::cos(a);
-                w[(size_t)(2 * m + 1)] = -std::sin(a);
-            }
-            const size_t bytes = (size_t)k * p->cx_size();
-            std::vector<float> wf;
-            const void* src = w.data();
-            if (!precision) {
-                wf.assign(w.begin(), w.end());
-                src = wf.data();
-            }
-            if (hipMalloc(&p->d_tw[d], bytes) != hipSuccess ||
-                hipMemcpy(p->d_tw[d], src, bytes, hipMemcpyHostToDevice) != hipSuccess) {
-                mi355nufft_plan_destroy(p);
-                return fail(MI355NUFFT_ENOMEM, "FFT twiddle table");
-            }
+        const int rc = make_twiddles(ndim, Kd, precision, p->d_tw);
+        if (rc) {
+            mi355nufft_plan_destroy(p);
+            return rc;
         }
     }
     *plan = p;
@@ -1490,6 +1633,88 @@ int mi355nufft_table1_adjoint(int ndim, const int64_t* Kd, const int32_t* Jd, in
     RET_IF(zero_grid<float>(ck, nk * nreps, s));
     return run_table1<float>(true, ndim, Kd, Jd, L, complex_kernel, fm, nullptr, nullptr, ck, h, tm, M, nreps,
                              nullptr, s);
+}
+
+int mi355nufft_toeplitz_create(mi355nufft_toeplitz** out, int ndim, const int64_t* Nd, int precision,
+                               void* stream) {
+    g_err.clear();
+    if (!out) return fail(MI355NUFFT_EINVAL, "output pointer is null");
+    *out = nullptr;
+    if (ndim < 1 || ndim > 3) return fail(MI355NUFFT_EINVAL, "ndim must be 1, 2 or 3 (got %d)", ndim);
+    if (!Nd) return fail(MI355NUFFT_EINVAL, "Nd is required");
+    if (precision != 0 && precision != 1) return fail(MI355NUFFT_EINVAL, "precision must be 0 or 1");
+    mi355nufft_toeplitz* tp = new mi355nufft_toeplitz();
+    tp->ndim = ndim;
+    tp->precision = precision;
+    tp->stream = (hipStream_t)stream;
+    for (int d = 0; d < ndim; ++d) {
+        if (Nd[d] < 1) {
+            delete tp;
+            return fail(MI355NUFFT_EINVAL, "Nd[%d] must be positive", d);
+        }
+        int64_t k = kFftMinK;
+        while (k < 2 * Nd[d]) k *= 2;
+        tp->N[d] = Nd[d];
+        tp->K[d] = k;
+        tp->nk *= k;
+    }
+    if (!fused_fft_sizes_ok(ndim, tp->K, tp->cx_size())) {
+        delete tp;
+        return fail(MI355NUFFT_EINVAL, "circulant grid beyond the fused FFT sizes (2 N_d <= %d, complex128 %d)",
+                    kFftMaxK, kFftMaxK / 2);
+    }
+    if (make_twiddles(ndim, tp->K, precision, tp->d_tw) ||
+        hipMalloc(&tp->d_T, (size_t)tp->nk * tp->cx_size()) != hipSuccess) {
+        for (int d = 0; d < 3; ++d)
+            if (tp->d_tw[d]) (void)hipFree(tp->d_tw[d]);
+        delete tp;
+        return fail(MI355NUFFT_ENOMEM, "Toeplitz plan storage");
+    }
+    *out = tp;
+    return MI355NUFFT_OK;
+}
+
+int mi355nufft_toeplitz_destroy(mi355nufft_toeplitz* tp) {
+    if (!tp) return MI355NUFFT_OK;
+    (void)hipStreamSynchronize(tp->stream);
+    for (int d = 0; d < 3; ++d)
+        if (tp->d_tw[d]) (void)hipFree(tp->d_tw[d]);
+    if (tp->d_T) (void)hipFree(tp->d_T);
+    if (tp->d_grid) (void)hipFree(tp->d_grid);
+    delete tp;
+    return MI355NUFFT_OK;
+}
+
+int mi355nufft_toeplitz_set_stream(mi355nufft_toeplitz* tp, void* stream) {
+    g_err.clear();
+    if (!tp) return fail(MI355NUFFT_EINVAL, "null Toeplitz plan");
+    tp->stream = (hipStream_t)stream;
+    return MI355NUFFT_OK;
+}
+
+int mi355nufft_toeplitz_grid(const mi355nufft_toeplitz* tp, int64_t* Kd) {
+    g_err.clear();
+    if (!tp || !Kd) return fail(MI355NUFFT_EINVAL, "null argument");
+    for (int d = 0; d < tp->ndim; ++d) Kd[d] = tp->K[d];
+    return MI355NUFFT_OK;
+}
+
+int mi355nufft_toeplitz_set_psf(mi355nufft_toeplitz* tp, const void* psf, const int64_t* ps) {
+    g_err.clear();
+    if (!tp) return fail(MI355NUFFT_EINVAL, "null Toeplitz plan");
+    if (!psf || !ps) return fail(MI355NUFFT_EINVAL, "psf and ps are required");
+    return tp->precision ? toeplitz_set_psf_impl<double>(tp, psf, ps) : toeplitz_set_psf_impl<float>(tp, psf, ps);
+}
+
+int mi355nufft_toeplitz_apply(mi355nufft_toeplitz* tp, const void* x, const int64_t* xs, void* y,
+                              const int64_t* ys, int32_t nimg) {
+    g_err.clear();
+    if (!tp) return fail(MI355NUFFT_EINVAL, "null Toeplitz plan");
+    RET_IF(check_nreps(nimg));
+    if (!x || !xs || !y || !ys) return fail(MI355NUFFT_EINVAL, "x, xs, y and ys are required");
+    if (!tp->have_T) return fail(MI355NUFFT_ESTATE, "mi355nufft_toeplitz_set_psf was not called");
+    return tp->precision ? toeplitz_apply_impl<double>(tp, x, xs, y, ys, nimg)
+                         : toeplitz_apply_impl<float>(tp, x, xs, y, ys, nimg);
 }
 
 }  // extern "C"
