@@ -20,7 +20,7 @@ import torch
 
 from . import _lib
 
-__all__ = ["tune", "candidates", "cache_path"]
+__all__ = ["tune", "tune_fft", "candidates", "cache_path"]
 
 # (bin size, points per subproblem, threads per workgroup)
 _C2 = [((b, b), p, t) for b in (16, 32) for p in (64, 128, 256, 512) for t in (64, 128, 256) if p >= t]
@@ -79,6 +79,27 @@ def _time(fn, reps):
     return best
 
 
+def tune_fft(op, nreps=1, reps=20):
+    """Choose the grid FFT of ``op`` ("auto": pruned passes fused with the
+    deapodization, or "rocfft") by timing a whole forward + adjoint with
+    each, and leave the plan set to the faster.  Needs the points set;
+    returns (choice, {choice: ms})."""
+    op.set_fft("auto")
+    if not op.plan_info()["fused_fft"]:  # no fused passes for this grid: auto is rocFFT
+        return "auto", {}
+    x = torch.randn(tuple(op.Nd) + (nreps,), dtype=op._tcdt, device=op.device)
+    out = {}
+    for m in ("auto", "rocfft"):
+        op.set_fft(m)
+
+        def pair():
+            op._adjoint_dev(op._forward_dev(x, nreps), nreps)
+        out[m] = _time(pair, reps)
+    best = min(out, key=out.get)
+    op.set_fft(best)
+    return best, out
+
+
 def tune(op, nreps=1, cands=None, reps=3, use_cache=True, verbose=False):
     """Choose bin size / points per subproblem for ``op`` (a NufftBase, or
     the ``.op`` of a NufftFrames) with ``nreps`` repetitions per frame.
@@ -91,6 +112,7 @@ def tune(op, nreps=1, cands=None, reps=3, use_cache=True, verbose=False):
     if use_cache and key in table:
         b, p = table[key]["bin_size"], table[key]["max_points"]
         op.set_binning(tuple(b), p, "auto", table[key].get("threads", 0))
+        op.set_fft(table[key].get("fft", "auto"))
         return tuple(b), p, table[key].get("timings", {})
     cands = cands or candidates(op.ndim)
     nimg = getattr(op, "nframes", 1) * nreps
@@ -122,7 +144,10 @@ def tune(op, nreps=1, cands=None, reps=3, use_cache=True, verbose=False):
         return None, 0, timings
     _, b, p, th = best
     op.set_binning(b, p, "auto", th)
+    fft, fft_ms = ("auto", {}) if getattr(op, "nframes", 1) > 1 else tune_fft(op, nreps)
+    if verbose:
+        print("autotune fft", fft, fft_ms)
     if use_cache:
-        table[key] = {"bin_size": list(b), "max_points": p, "threads": th, "timings": timings}
+        table[key] = {"bin_size": list(b), "max_points": p, "threads": th, "timings": timings, "fft": fft}
         _store(table)
     return tuple(b), p, timings
