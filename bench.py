@@ -35,7 +35,7 @@ KERNEL_NAMES = {"interp": "interp_tiled_kernel<float, 2, false, 6, true>",
 # FETCH_SIZE and WRITE_SIZE passes over this bench (--calibrate-copy), each
 # scaled by what the counter reads for a 256 MiB device copy (FETCH_SIZE
 # x 2.0, WRITE_SIZE x 1.0); profiles/nufft2d_256_c64_hbm_traffic.md.
-TRAFFIC_BYTES = {"spread": 20458188, "interp": 14012109}
+TRAFFIC_BYTES = {"spread": 20424294, "interp": 13992141}
 
 
 def parse():
@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--no-extra", action="store_true", help="skip the complex128 / 3-D extra measurements")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--graph", action="store_true", help="time a HIP-graph replay instead of eager launches")
+    ap.add_argument("--fft", choices=("tuned", "fused", "rocfft"), default="tuned",
+                    help="grid FFT: the faster of the two measured here (default), the fused pruned passes, or rocFFT")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run three 256 MiB device-to-device copies (a known-size reference for "
                          "FETCH_SIZE / WRITE_SIZE counter passes)")
@@ -311,7 +313,11 @@ def main():
     # grid FFT of the plan: the fused pruned passes or rocFFT, whichever
     # times faster here for this geometry (autotune.tune_fft)
     from mrrt.nufft_amd import autotune
-    fft_choice, fft_ms = autotune.tune_fft(run.A, nreps=1)
+    if args.fft == "tuned":
+        fft_choice, fft_ms = autotune.tune_fft(run.A, nreps=1)
+    else:
+        fft_choice, fft_ms = ("auto" if args.fft == "fused" else "rocfft"), {}
+        run.A.set_fft(fft_choice)
     print("bench: plan ready (M=%d, fft %s %s); timing %d steps after %d warmup"
           % (run.M, fft_choice, json.dumps({k: round(v, 5) for k, v in fft_ms.items()}), args.steps, args.warmup),
           file=sys.stderr, flush=True)
