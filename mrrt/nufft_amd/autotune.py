@@ -79,22 +79,26 @@ def _time(fn, reps):
     return best
 
 
-def tune_fft(op, nreps=1, reps=20):
+def tune_fft(op, nreps=1, pairs=20, rounds=3):
     """Choose the grid FFT of ``op`` ("auto": pruned passes fused with the
-    deapodization, or "rocfft") by timing a whole forward + adjoint with
-    each, and leave the plan set to the faster.  Needs the points set;
-    returns (choice, {choice: ms})."""
+    deapodization, or "rocfft") by timing back-to-back forward + adjoint
+    pairs with each (``rounds`` alternating rounds of ``pairs`` pairs, best
+    round per path), and leave the plan set to the faster.  Needs the points
+    set; returns (choice, {choice: ms per pair})."""
     op.set_fft("auto")
     if not op.plan_info()["fused_fft"]:  # no fused passes for this grid: auto is rocFFT
         return "auto", {}
     x = torch.randn(tuple(op.Nd) + (nreps,), dtype=op._tcdt, device=op.device)
-    out = {}
-    for m in ("auto", "rocfft"):
-        op.set_fft(m)
 
-        def pair():
+    def run():
+        for _ in range(pairs):
             op._adjoint_dev(op._forward_dev(x, nreps), nreps)
-        out[m] = _time(pair, reps)
+    out = {}
+    for _ in range(rounds):
+        for m in ("auto", "rocfft"):
+            op.set_fft(m)
+            t = _time(run, 1) / pairs
+            out[m] = min(out.get(m, float("inf")), t)
     best = min(out, key=out.get)
     op.set_fft(best)
     return best, out

@@ -203,6 +203,7 @@ struct TiledArgs {
     cx<T>* tiles;           // scatter, two-phase: (sub, rep) tile slots; null = atomic flush
     const T* tms;           // tm in sorted order: tms[d*M + i] = tm[d*M + perm[i]]
     const cx<T>* phs;       // per-sample phase in sorted order, or null
+    int rc;                 // gather: repetitions staged together (LDS holds rc tiles)
 };
 
 // Copy tm and the per-sample phase into sorted (bin) order, so the tiled
@@ -429,10 +430,12 @@ __device__ __forceinline__ void scatter_taps(double* tre, double* tim, const Poi
     }
 }
 
-template <typename T, int NDIM, bool CK, int JMAX, bool FIXED>
+// MULTI: repetitions (coils) are processed a.rc at a time: a.rc tiles are
+// staged, and each point's coefficients are computed once for all of them.
+template <typename T, int NDIM, bool CK, int JMAX, bool FIXED, bool MULTI>
 __global__ __launch_bounds__(256) void interp_tiled_kernel(TiledArgs<T, CK> a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    cx<T>* tile = reinterpret_cast<cx<T>*>(smem);
+    cx<T>* tiles = reinterpret_cast<cx<T>*>(smem);
     const InterpArgs<T, CK>& ia = a.ia;
     const BinGeom& bg = a.bg;
     const AxisGeom ax[3] = {ia.ax[0], ia.ax[1], ia.ax[2]};
@@ -442,13 +445,31 @@ __global__ __launch_bounds__(256) void interp_tiled_kernel(TiledArgs<T, CK> a) {
     for (int sid = blockIdx.x; sid < n_sub; sid += gridDim.x) {
         Sub s;
         load_sub(bg, a.bs, NDIM, sid, s);
-        for (int r = 0; r < ia.nreps; ++r) {
-            stage_tile<T, NDIM, CK>(a, s, ax, ia.grid + ((int64_t)s.frame * ia.nreps + r) * ia.kstride, tile);
+        if constexpr (!MULTI) {
+            for (int r = 0; r < ia.nreps; ++r) {
+                stage_tile<T, NDIM, CK>(a, s, ax, ia.grid + ((int64_t)s.frame * ia.nreps + r) * ia.kstride, tiles);
+                __syncthreads();
+                for (int i = s.start + tid; i < s.end; i += blockDim.x) {
+                    PointTaps<T, CK, JMAX> pt;
+                    point_taps<T, NDIM, CK, JMAX, FIXED>(a, ax, s, i, pt);
+                    ia.out[(int64_t)r * ia.M + pt.m] = gather_taps<T, NDIM, CK, JMAX, FIXED>(tiles, pt, ax, T0, T1, a.phs);
+                }
+                __syncthreads();
+            }
+            continue;
+        }
+        for (int r0 = 0; r0 < ia.nreps; r0 += a.rc) {
+            const int rc = min(a.rc, ia.nreps - r0);
+            for (int rr = 0; rr < rc; ++rr)
+                stage_tile<T, NDIM, CK>(a, s, ax, ia.grid + ((int64_t)s.frame * ia.nreps + r0 + rr) * ia.kstride,
+                                        tiles + (int64_t)rr * bg.tile);
             __syncthreads();
             for (int i = s.start + tid; i < s.end; i += blockDim.x) {
                 PointTaps<T, CK, JMAX> pt;
                 point_taps<T, NDIM, CK, JMAX, FIXED>(a, ax, s, i, pt);
-                ia.out[(int64_t)r * ia.M + pt.m] = gather_taps<T, NDIM, CK, JMAX, FIXED>(tile, pt, ax, T0, T1, a.phs);
+                for (int rr = 0; rr < rc; ++rr)
+                    ia.out[(int64_t)(r0 + rr) * ia.M + pt.m] =
+                        gather_taps<T, NDIM, CK, JMAX, FIXED>(tiles + (int64_t)rr * bg.tile, pt, ax, T0, T1, a.phs);
             }
             __syncthreads();
         }
@@ -463,6 +484,8 @@ __device__ __forceinline__ cx<T> sample_times_conj_phase(const InterpArgs<T, CK>
     return mkcx<T>(f.x * pt.ph.x + f.y * pt.ph.y, f.y * pt.ph.x - f.x * pt.ph.y);
 }
 
+// One repetition at a time: staging several repetitions' float64 tiles was
+// measured no faster (the LDS atomics dominate) and costs occupancy.
 template <typename T, int NDIM, bool CK, int JMAX, bool FIXED>
 __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
     extern __shared__ __align__(16) unsigned char smem[];

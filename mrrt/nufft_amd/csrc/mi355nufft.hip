@@ -929,20 +929,24 @@ int timing_events(mi355nufft_plan* p, int w, hipEvent_t* start, hipEvent_t* stop
 template <typename T, bool CK, int NDIM, int JMAX, bool FIXED>
 void launch_tiled(bool adjoint, const TiledArgs<T, CK>& a, unsigned nsub, unsigned threads, size_t lds,
                   hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+    auto spread = spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED>;
+    auto interp = a.rc > 1 ? interp_tiled_kernel<T, NDIM, CK, JMAX, FIXED, true>
+                           : interp_tiled_kernel<T, NDIM, CK, JMAX, FIXED, false>;
     if (ev0) {
         if (adjoint)
-            hipExtLaunchKernelGGL((spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED>), dim3(nsub), dim3(threads),
-                                  (uint32_t)lds, s, ev0, ev1, 0u, a);
+            hipExtLaunchKernelGGL(spread, dim3(nsub), dim3(threads), (uint32_t)lds, s, ev0, ev1, 0u, a);
         else
-            hipExtLaunchKernelGGL((interp_tiled_kernel<T, NDIM, CK, JMAX, FIXED>), dim3(nsub), dim3(threads),
-                                  (uint32_t)lds, s, ev0, ev1, 0u, a);
+            hipExtLaunchKernelGGL(interp, dim3(nsub), dim3(threads), (uint32_t)lds, s, ev0, ev1, 0u, a);
         return;
     }
     if (adjoint)
-        hipLaunchKernelGGL((spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED>), dim3(nsub), dim3(threads), lds, s, a);
+        hipLaunchKernelGGL(spread, dim3(nsub), dim3(threads), lds, s, a);
     else
-        hipLaunchKernelGGL((interp_tiled_kernel<T, NDIM, CK, JMAX, FIXED>), dim3(nsub), dim3(threads), lds, s, a);
+        hipLaunchKernelGGL(interp, dim3(nsub), dim3(threads), lds, s, a);
 }
+
+// Repetitions (coils) whose tiles one tiled launch stages together.
+constexpr int kMaxRepChunk = 8;
 
 // Taps compiled as fixed trip counts when every axis uses the same J among
 // these; other widths run the generic kernels (J <= JMAX checked per tap).
@@ -974,7 +978,15 @@ int run_tiled(mi355nufft_plan* p, bool adjoint, const void* samples_in, void* sa
     a.tms = (const T*)p->d_tms;
     a.phs = p->has_phase ? (const cx<T>*)p->d_phs : nullptr;
     const unsigned nsub = tiled_grid(p);
-    const size_t lds = adjoint ? tile_bytes(p) : gather_tile_bytes(p);
+    // the gather stages up to kMaxRepChunk repetitions' tiles at once
+    // (coefficients are then computed once per point for all of them),
+    // within the LDS budget; the scatter one at a time
+    const size_t one = adjoint ? tile_bytes(p) : gather_tile_bytes(p);
+    int rc = adjoint ? 1 : (int)(p->lds_limit / one);
+    rc = rc < kMaxRepChunk ? rc : kMaxRepChunk;
+    rc = rc < nreps ? rc : nreps;
+    a.rc = rc > 1 ? rc : 1;
+    const size_t lds = one * (size_t)a.rc;
     const int jm = jmax_of(p->ndim, J);
     const int jf = fixed_j(p->ndim, J);
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1029,6 +1041,7 @@ int run_grad(mi355nufft_plan* p, const void* grid, const void* w, void* grad, in
     a.scale_ortho = (T)p->scale_ortho;
     a.tms = (const T*)p->d_tms;
     a.phs = p->has_phase ? (const cx<T>*)p->d_phs : nullptr;
+    a.rc = 1;
     GradArgs<T> g;
     g.w = (const cx<T>*)w;
     g.grad = (T*)grad;

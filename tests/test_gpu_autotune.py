@@ -45,7 +45,7 @@ def test_tune_keeps_results_and_caches(tmp_path, monkeypatch, precision, tol):
 def test_tune_fft_picks_a_path_and_sets_it():
     Nd, Kd = (32, 32), (64, 64)
     A = NufftBase(Nd, trj.golden_angle_2d(30, 64), Jd=6, Kd=Kd, Ld=1024, device=dev())
-    choice, ms = autotune.tune_fft(A, nreps=1, reps=3)
+    choice, ms = autotune.tune_fft(A, nreps=1, pairs=3, rounds=2)
     assert choice in ("auto", "rocfft") and set(ms) == {"auto", "rocfft"}
     assert A.plan_info()["fused_fft"] == (choice == "auto")
     C = NufftBase((20, 30), trj.golden_angle_2d(30, 48), Jd=6, Kd=(48, 64), Ld=1024, device=dev())
