@@ -204,12 +204,32 @@ def run_c4(steps=3, autotune=True):
 
     t_b = _time_fn(batched, steps, 1)
     t_l = _time_fn(loop, steps, 1)
+    # stage split of one batched forward + adjoint (plan stages, 768 images)
+    import ctypes
+    from mrrt.nufft_amd import _lib
+    op, lib, vp = B.op, B.op._lib, ctypes.c_void_p
+    xv = B._image_view(x)
+    xs = _lib.i64arr(list(xv.stride()))
+    g = B._grid
+    nimg = F * C
+    op._sync_stream()
+    stages = {
+        "forward_grid": lambda: _lib.check(lib.mi355nufft_forward_grid(op._plan, vp(xv.data_ptr()), xs,
+                                                                       vp(g.data_ptr()), nimg)),
+        "interp": lambda: _lib.check(lib.mi355nufft_interp(op._plan, vp(g.data_ptr()), vp(y.data_ptr()), C)),
+        "spread": lambda: _lib.check(lib.mi355nufft_spread(op._plan, vp(y.data_ptr()), vp(g.data_ptr()), C)),
+        "adjoint_image": lambda: _lib.check(lib.mi355nufft_adjoint_image(op._plan, vp(g.data_ptr()),
+                                                                         vp(xv.data_ptr()), xs, nimg)),
+    }
+    st = {k: round(_time_fn(fn, steps, 1), 4) for k, fn in stages.items()}
     b_bytes = sum(interp_bytes(om.shape[0], 2, 512 * 512, C, 4) for om in frames)
     return {"c4_frames_x_coils": [F, C], "c4_M_total": int(B.M),
             "c4_batched_graph_ms_per_pair": round(t_b, 3), "c4_per_frame_loop_ms_per_pair": round(t_l, 3),
             "c4_speedup_batched_vs_loop": round(t_l / t_b, 3),
             "c4_transforms_per_s_batched": round(2 * F * C * 1000.0 / t_b, 1),
-            "c4_interp_bytes_per_transform": int(b_bytes), "c4_autotuned": tuned}
+            "c4_interp_bytes_per_transform": int(b_bytes), "c4_autotuned": tuned, "c4_stages_ms": st,
+            "c4_interp_GBps": round(b_bytes / (st["interp"] * 1e-3) / 1e9, 1),
+            "c4_spread_GBps": round(b_bytes / (st["spread"] * 1e-3) / 1e9, 1)}
 
 
 def run_c5(steps=5):

@@ -204,6 +204,7 @@ struct TiledArgs {
     const T* tms;           // tm in sorted order: tms[d*M + i] = tm[d*M + perm[i]]
     const cx<T>* phs;       // per-sample phase in sorted order, or null
     int rc;                 // gather: repetitions staged together (LDS holds rc tiles)
+    int own_ok;             // scatter: every K_d is a multiple of B_d (see spread_tiled_kernel)
 };
 
 // Copy tm and the per-sample phase into sorted (bin) order, so the tiled
@@ -223,6 +224,7 @@ struct Sub {
     int start, end;
     int frame;  // frame of the bin (0 without frames)
     int o[3];   // tile origin (grid coordinates)
+    int solo;   // the only subproblem of its bin
 };
 
 // Subproblem sid; the workgroups of a launch stride over the subproblem list
@@ -231,6 +233,7 @@ __device__ __forceinline__ void load_sub(const BinGeom& bg, const BinState& bs, 
     const int bin = bs.sub_bin[sid];
     s.start = bs.sub_start[sid];
     s.end = bs.sub_end[sid];
+    s.solo = bs.counts[bin] <= bg.P;
     s.frame = bin / bg.nbins_frame;
     int q = bin - s.frame * bg.nbins_frame;
     for (int d = 0; d < 3; ++d) {
@@ -240,17 +243,24 @@ __device__ __forceinline__ void load_sub(const BinGeom& bg, const BinState& bs, 
     }
 }
 
-// grid index of local tile point e
+// grid index of local tile point e (local coordinates e0, e1, e2)
 __device__ __forceinline__ int64_t tile_to_grid(int e, const BinGeom& bg, const Sub& s, const AxisGeom* ax,
-                                                int& g0, int& g1, int& g2) {
-    const int e0 = e % bg.T[0];
+                                                int& g0, int& g1, int& g2, int& e0, int& e1, int& e2) {
+    e0 = e % bg.T[0];
     const int q = e / bg.T[0];
-    const int e1 = q % bg.T[1];
-    const int e2 = q / bg.T[1];
+    e1 = q % bg.T[1];
+    e2 = q / bg.T[1];
     g0 = (s.o[0] + e0) % ax[0].K;
     g1 = (s.o[1] + e1) % ax[1].K;
     g2 = (s.o[2] + e2) % ax[2].K;
     return ((int64_t)g2 * ax[1].K + g1) * ax[0].K + g0;
+}
+
+// grid index of local tile point e
+__device__ __forceinline__ int64_t tile_to_grid(int e, const BinGeom& bg, const Sub& s, const AxisGeom* ax,
+                                                int& g0, int& g1, int& g2) {
+    int e0, e1, e2;
+    return tile_to_grid(e, bg, s, ax, g0, g1, g2, e0, e1, e2);
 }
 
 // Stage the tile of subproblem s from grid g into LDS: grid * phase_before
@@ -524,12 +534,18 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
                 continue;
             }
             cx<T>* __restrict__ g = ia.grid_out + ((int64_t)s.frame * ia.nreps + r) * ia.kstride;
+            // A bin's own cells past the previous bin's halo (local e_d in
+            // [J_d - 1, B_d) on every axis) lie in no other tile when every
+            // K_d is a multiple of B_d; when the bin has a single
+            // subproblem they are written with plain stores (the grid is
+            // zero there), everything else with atomics.
+            const bool own = a.own_ok && s.solo;
             for (int e = tid; e < bg.tile; e += blockDim.x) {
                 const double ar = tre[e], ai = tim[e];
                 if (ar == 0.0 && ai == 0.0) continue;
                 T vr = (T)ar, vi = (T)ai;
-                int g0, g1, g2;
-                const int64_t gi = tile_to_grid(e, bg, s, ax, g0, g1, g2);
+                int g0, g1, g2, e0, e1, e2;
+                const int64_t gi = tile_to_grid(e, bg, s, ax, g0, g1, g2, e0, e1, e2);
                 if (a.use_pb) {  // * conj(phase_before)
                     const T c[3] = {a.pb[0], a.pb[1], a.pb[2]};
                     const cx<T> p = cis_tile(phase_before_arg<T>(c, NDIM, g0, g1, g2));
@@ -538,9 +554,16 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
                     vr = wr;
                     vi = wi;
                 }
-                T* q = reinterpret_cast<T*>(g + gi);
-                unsafeAtomicAdd(q, vr);
-                unsafeAtomicAdd(q + 1, vi);
+                bool mine = own && e0 >= ax[0].J - 1 && e0 < bg.B[0];
+                if (NDIM > 1) mine = mine && e1 >= ax[1].J - 1 && e1 < bg.B[1];
+                if (NDIM > 2) mine = mine && e2 >= ax[2].J - 1 && e2 < bg.B[2];
+                if (mine) {
+                    g[gi] = mkcx<T>(vr, vi);
+                } else {
+                    T* q = reinterpret_cast<T*>(g + gi);
+                    unsafeAtomicAdd(q, vr);
+                    unsafeAtomicAdd(q + 1, vi);
+                }
             }
             __syncthreads();
         }

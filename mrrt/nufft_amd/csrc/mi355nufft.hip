@@ -977,6 +977,9 @@ int run_tiled(mi355nufft_plan* p, bool adjoint, const void* samples_in, void* sa
     a.tiles = adjoint ? (cx<T>*)p->tiles_target : nullptr;
     a.tms = (const T*)p->d_tms;
     a.phs = p->has_phase ? (const cx<T>*)p->d_phs : nullptr;
+    a.own_ok = 1;
+    for (int d = 0; d < p->ndim; ++d)
+        if (p->K[d] % p->bg.B[d] != 0) a.own_ok = 0;
     const unsigned nsub = tiled_grid(p);
     // the gather stages up to kMaxRepChunk repetitions' tiles at once
     // (coefficients are then computed once per point for all of them),
