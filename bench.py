@@ -312,7 +312,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every run here uses a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not measured)", file=sys.stderr)
+              "(the path shards by frames and coils, or by slab; only one GPU is measured)", file=sys.stderr)
         sys.exit(2)
     import torch
     if not torch.cuda.is_available():

@@ -12,7 +12,7 @@ nufft_oracle
     (mrrt/nufft/_nufft.py, _kaiser_bessel.py, _kernels.py, _utils.py,
     _dtft.py and the C kernels in _extensions/c/nufft_table.template.c).
 ref_ctypes
-    ctypes driver for ``oracle/_ref/libnufft_table_ref.so``: the reference's
+    ctypes bindings for ``oracle/_ref/libnufft_table_ref.so``: the reference's
     own C kernels compiled from /root/reference by ``oracle/build_ref.sh``.
 
 Pinning status of each function is stated in the module docstrings.

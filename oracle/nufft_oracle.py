@@ -4,7 +4,7 @@ TEST INFRASTRUCTURE ONLY -- see ``oracle/__init__.py``.
 
 Every function cites the reference file:line it restates (paths relative to
 the reference tree, package ``mrrt/nufft``).  The float32 quirks of the
-reference are kept on purpose, because parity is judged against the
+reference are kept on purpose, because parity is measured against the
 reference's own output:
 
 * the interpolation table is built through a float32 sparse 1-D NUFFT

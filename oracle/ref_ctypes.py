@@ -1,4 +1,4 @@
-"""ctypes driver for the reference's own table1 C kernels (oracle/_ref).
+"""ctypes bindings for the reference's own table1 C kernels (oracle/_ref).
 
 TEST INFRASTRUCTURE ONLY -- see ``oracle/__init__.py``.
 
