@@ -190,10 +190,10 @@ int mi355nufft_forward(mi355nufft_plan *plan, const void *x, const int64_t *xs,
  *   y          device complex (nreps, M)
  *   x          device complex image batch with element strides xs, written
  *   grid       device scratch of nreps*prod(Kd) complex (overwritten), or
- *              NULL to use a plan-owned buffer.  With the fused FFT passes
- *              the plan-owned buffer is left all zero by each adjoint, so
- *              the next adjoint skips clearing it (except under stream
- *              capture, where it is always cleared).
+ *              NULL to use a plan-owned buffer.  The last FFT stage writes
+ *              zeros over the plan-owned buffer as it reads it, so each
+ *              adjoint leaves it all zero and the next one skips clearing
+ *              it (except under stream capture, where it is always cleared).
  */
 int mi355nufft_adjoint(mi355nufft_plan *plan, const void *y, void *x,
                        const int64_t *xs, int32_t nreps, void *grid);

@@ -430,7 +430,7 @@ class NufftBase(object):
             out = torch.empty((nreps,) + tuple(reversed(self._Nd)), dtype=self._tcdt, device=self.device)
             out = out.permute(*(tuple(range(self.ndim, 0, -1)) + (0,)))
         self._sync_stream()
-        # plan-owned grid: kept zero between adjoints by the fused FFT passes
+        # plan-owned grid: left zero by each adjoint's last FFT stage
         _lib.check(self._lib.mi355nufft_adjoint(
             self._plan, ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(out.data_ptr()),
             _lib.i64arr(list(out.stride())), nreps, None))

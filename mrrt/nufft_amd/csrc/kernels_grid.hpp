@@ -122,6 +122,30 @@ __global__ __launch_bounds__(256) void crop_deapod_kernel(const cx<T>* __restric
     x[(int64_t)n1 * g.xs[0] + (int64_t)n2 * g.xs[1] + (int64_t)n3 * g.xs[2] + (int64_t)r * g.xs[3]] = o;
 }
 
+// The crop of crop_deapod_kernel over the whole grid, writing zeros over
+// every grid point after reading it (the grid is left all zero).
+template <typename T>
+__global__ __launch_bounds__(256) void crop_deapod_rezero_kernel(cx<T>* __restrict__ grid, cx<T>* __restrict__ x,
+                                                                 GridGeom g, const double* sn0, const double* sn1,
+                                                                 const double* sn2, T scale) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.nk) return;
+    const int r = blockIdx.y;
+    cx<T>* gp = grid + (int64_t)r * g.nk + i;
+    int k1, k2, k3;
+    split_index(i, g, k1, k2, k3);
+    if (k1 < g.N[0] && k2 < g.N[1] && k3 < g.N[2]) {
+        const double* sn[3] = {sn0, sn1, sn2};
+        const T s = deapod_factor<T>(sn, g.ndim, k1, k2, k3);
+        const cx<T> v = *gp;
+        cx<T> o;
+        o.x = (v.x * scale) * s;
+        o.y = (v.y * scale) * s;
+        x[(int64_t)k1 * g.xs[0] + (int64_t)k2 * g.xs[1] + (int64_t)k3 * g.xs[2] + (int64_t)r * g.xs[3]] = o;
+    }
+    *gp = mkcx<T>((T)0, (T)0);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void zero_kernel(cx<T>* __restrict__ p, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -429,10 +429,18 @@ int phase_grid_impl(mi355nufft_plan* p, void* grid, int32_t nreps, int with_phas
 }
 
 template <typename T>
-int crop_deapod_impl(mi355nufft_plan* p, const void* grid, void* x, const int64_t* xs, int32_t nreps) {
+int crop_deapod_impl(mi355nufft_plan* p, const void* grid, void* x, const int64_t* xs, int32_t nreps,
+                     int rezero = 0) {
     GridGeom g = grid_geom(p, xs);
     double scale = p->adjoint_scale;
     if (p->ortho) scale = p->scale_ortho * p->adjoint_scale / (double)p->nk;
+    if (rezero) {
+        hipLaunchKernelGGL((crop_deapod_rezero_kernel<T>), dim3(nblocks(p->nk), nreps), dim3(kBlock), 0, p->stream,
+                           (cx<T>*)grid, (cx<T>*)x, g, p->d_sn[0], p->ndim > 1 ? p->d_sn[1] : p->d_sn[0],
+                           p->ndim > 2 ? p->d_sn[2] : p->d_sn[0], (T)scale);
+        LAUNCH_CHECK();
+        return MI355NUFFT_OK;
+    }
     hipLaunchKernelGGL((crop_deapod_kernel<T>), dim3(nblocks(p->nn), nreps), dim3(kBlock), 0, p->stream,
                        (const cx<T>*)grid, (cx<T>*)x, g, p->nn, p->d_sn[0],
                        p->ndim > 1 ? p->d_sn[1] : p->d_sn[0], p->ndim > 2 ? p->d_sn[2] : p->d_sn[0], (T)scale);
@@ -648,13 +656,12 @@ int forward_grid_impl(mi355nufft_plan* p, const void* x, const int64_t* xs, void
 
 // oversampled grid -> image: ifftn, crop to Nd, scale, * conj(sn)
 // (_nufft.py:1524-1572); the conj(phase_before) of :1519 is applied by the
-// scatter.  The grid is overwritten; with rezero (fused passes only) it is
-// left all zero.
+// scatter.  The grid is overwritten; with rezero it is left all zero.
 template <typename T>
 int adjoint_image_impl(mi355nufft_plan* p, void* grid, void* x, const int64_t* xs, int32_t nimg, int rezero) {
     if (!use_fused_fft(p)) {
         RET_IF(fft_impl(p, grid, nimg, 1));
-        return crop_deapod_impl<T>(p, grid, x, xs, nimg);
+        return crop_deapod_impl<T>(p, grid, x, xs, nimg, rezero);
     }
     const FftDev f = fft_dev(p);
     FftPassOpts o;
@@ -1595,7 +1602,7 @@ int mi355nufft_adjoint(mi355nufft_plan* p, const void* y, void* x, const int64_t
     RET_IF(get_agrid(p, nimg));
     const bool capturing = stream_capturing(p);
     const bool was_zero = p->agrid_zero && !capturing;
-    const int rezero = use_fused_fft(p) ? 1 : 0;
+    const int rezero = 1;
     p->agrid_zero = 0;
     RET_IF(need_points(p));
     if (!y && p->M > 0) return fail(MI355NUFFT_EINVAL, "y is required");
