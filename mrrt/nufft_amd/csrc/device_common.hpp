@@ -63,6 +63,18 @@ __device__ __forceinline__ long long first_tap(T t, int J) {
     return 1 + (long long)kd;
 }
 
+// Workgroups are dispatched to the 8 XCDs of an MI355X round-robin by
+// block index, and each XCD has its own L2.  xcd_remap turns block b of n
+// into a logical index such that each XCD gets one contiguous range of
+// logical indices (x*q + min(x, r) + b/8 for x = b % 8, n = 8q + r), so
+// neighbouring work items, which share cache lines, share an L2.
+// It is a bijection of [0, n).
+__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned n) {
+    constexpr unsigned kXcds = 8;
+    const unsigned q = n / kXcds, r = n % kXcds, x = b % kXcds, j = b / kXcds;
+    return x * q + (x < r ? x : r) + j;
+}
+
 __device__ __forceinline__ int pos_mod(long long k, int K) {
     long long r = k % (long long)K;
     return (int)(r < 0 ? r + K : r);

@@ -299,8 +299,9 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_cols_kernel(cx<T>* __restr
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
     const int lpw = g.lpw;
     const int64_t nchunk = g.S / lpw;
-    const int64_t chunk = blockIdx.x % nchunk;
-    const int64_t o = blockIdx.x / nchunk;
+    const int64_t bid = xcd_remap(blockIdx.x, gridDim.x);  // neighbouring chunks share 128-byte lines
+    const int64_t chunk = bid % nchunk;
+    const int64_t o = bid / nchunk;
     const int64_t ibase = (o % g.A) * g.st_a + chunk * lpw;  // within one image
     const int64_t base = ibase + (o / g.A) * g.nk;
     const int li = threadIdx.x % lpw, t0 = threadIdx.x / lpw;  // load / store mapping
