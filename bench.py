@@ -35,7 +35,7 @@ KERNEL_NAMES = {"interp": "interp_tiled_kernel<float, 2, false, 6, true>",
 # FETCH_SIZE and WRITE_SIZE passes over this bench (--calibrate-copy), each
 # scaled by what the counter reads for a 256 MiB device copy (FETCH_SIZE
 # x 2.0, WRITE_SIZE x 1.0); profiles/nufft2d_256_c64_hbm_traffic.md.
-TRAFFIC_BYTES = {"spread": 20424294, "interp": 13992141}
+TRAFFIC_BYTES = {"spread": 20200960, "interp": 13982822}
 
 
 def parse():
