@@ -127,7 +127,8 @@ int64_t mi355nufft_num_points(const mi355nufft_plan *plan);
  *   bin_size    ndim bin sizes in grid points (NULL or 0 entries: defaults
  *               2-D 16x16, 3-D 16x8x8, 1-D 512; the scatter tile is
  *               accumulated in complex float64 and must fit the LDS budget)
- *   max_points  points per subproblem (0: default 256 in 2-D, 512 else)
+ *   max_points  points per subproblem (0: default 256 in 2-D, 1024 in 3-D,
+ *               512 in 1-D)
  *   method      0 auto (tiled when the tile fits in LDS), 1 global-memory
  *               kernels on unsorted points, 2 tiled (error if it cannot),
  *               3 tiled with the scatter flushed by global atomics,

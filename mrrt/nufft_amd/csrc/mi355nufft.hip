@@ -720,7 +720,7 @@ int fft_impl(mi355nufft_plan* p, void* grid, int32_t nreps, int direction) {
 
 // Default bin geometry (autotuned on golden-angle radial and kooshball
 // trajectories): 2-D 16x16 with P = 256 points per subproblem, 3-D 16x8x8
-// and 1-D 512 with P = 512.  Requested sizes override.
+// with P = 1024, 1-D 512 with P = 512.  Requested sizes override.
 void default_bins(const mi355nufft_plan* p, int B[3], int& P) {
     for (int d = 0; d < 3; ++d) B[d] = 1;
     if (p->ndim == 1) {
@@ -734,7 +734,7 @@ void default_bins(const mi355nufft_plan* p, int B[3], int& P) {
         B[0] = 16;
         B[1] = 8;
         B[2] = 8;
-        P = 512;
+        P = 1024;
     }
     for (int d = 0; d < p->ndim; ++d)
         if (p->bin_req[d] > 0) B[d] = p->bin_req[d];
@@ -952,8 +952,10 @@ void launch_tiled(bool adjoint, const TiledArgs<T, CK>& a, unsigned nsub, unsign
         hipLaunchKernelGGL(interp, dim3(nsub), dim3(threads), lds, s, a);
 }
 
-// Repetitions (coils) whose tiles one tiled launch stages together.
+// Repetitions (coils) whose tiles one tiled launch stages together, and the
+// LDS they may take (more per workgroup costs occupancy).
 constexpr int kMaxRepChunk = 8;
+constexpr size_t kRepChunkLds = 65536;
 
 // Taps compiled as fixed trip counts when every axis uses the same J among
 // these; other widths run the generic kernels (J <= JMAX checked per tap).
@@ -992,7 +994,8 @@ int run_tiled(mi355nufft_plan* p, bool adjoint, const void* samples_in, void* sa
     // (coefficients are then computed once per point for all of them),
     // within the LDS budget; the scatter one at a time
     const size_t one = adjoint ? tile_bytes(p) : gather_tile_bytes(p);
-    int rc = adjoint ? 1 : (int)(p->lds_limit / one);
+    const size_t chunk_lds = p->lds_limit < kRepChunkLds ? p->lds_limit : kRepChunkLds;
+    int rc = adjoint ? 1 : (int)(chunk_lds / one);
     rc = rc < kMaxRepChunk ? rc : kMaxRepChunk;
     rc = rc < nreps ? rc : nreps;
     a.rc = rc > 1 ? rc : 1;
@@ -1275,7 +1278,7 @@ int mi355nufft_plan_create(mi355nufft_plan** plan, int ndim, const int64_t* Nd, 
         int dev = 0, lds = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
             hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
-            p->lds_limit = (size_t)(lds < 65536 ? lds : 65536);
+            p->lds_limit = (size_t)lds;  // 160 KiB per workgroup on gfx950
     }
     for (int d = 0; d < ndim; ++d) {
         const size_t tab_bytes = (size_t)(Jd[d] * L + 1) * (phasing ? p->cx_size() : p->real_size());

@@ -384,9 +384,9 @@ def test_default_binning_active():
 
 def test_tiled_rejects_tile_over_lds_budget():
     """A forced tiled method with a tile beyond the LDS budget is an error."""
-    A = NufftBase((40, 36), trj.golden_angle_2d(10, 80), Jd=6, Kd=(80, 75), precision="double", device=dev())
+    A = NufftBase((80, 72), trj.golden_angle_2d(10, 160), Jd=6, Kd=(160, 150), precision="double", device=dev())
     with pytest.raises(ValueError, match="LDS"):
-        A.set_binning((200, 200), 64, "tiled")
+        A.set_binning((200, 200), 64, "tiled")  # 165 x 155 complex128 points: 409 KiB
 
 
 def test_graph_capture_plain_operator():
