@@ -362,7 +362,7 @@ __device__ __forceinline__ cx<T> gather_taps(const cx<T>* tile, const PointTaps<
                 madd<T, CK>(sr, si, pt.cr[1][j2], pt.ci[1][j2], s1r, s1i);
             }
     } else {
-#pragma unroll(FIXED ? JMAX : 1)
+#pragma unroll FIXED ? JMAX : 1
         for (int j3 = 0; j3 < JMAX; ++j3)
             if (FIXED || j3 < ax[2].J) {
                 T s2r = 0, s2i = 0;
@@ -419,7 +419,7 @@ __device__ __forceinline__ void scatter_taps(double* tre, double* tim, const Poi
                     }
             }
     } else {
-#pragma unroll(FIXED ? JMAX : 1)
+#pragma unroll FIXED ? JMAX : 1
         for (int j3 = 0; j3 < JMAX; ++j3)
             if (FIXED || j3 < ax[2].J) {
                 T v3r, v3i;
@@ -648,7 +648,7 @@ __global__ __launch_bounds__(256) void grad_tiled_kernel(TiledArgs<T, CK> a, Gra
                         cmadd<T, CK>(Sr[2], Si[2], dr[1][j2], di[1][j2], ar, ai);
                     }
             } else {
-#pragma unroll(FIXED ? JMAX : 1)
+#pragma unroll FIXED ? JMAX : 1
                 for (int j3 = 0; j3 < JMAX; ++j3)
                     if (FIXED || j3 < ax[2].J) {
                         T pr = 0, pi = 0, qr = 0, qi = 0, ur = 0, ui = 0;
