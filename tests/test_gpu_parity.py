@@ -120,6 +120,23 @@ def test_operator_1d_3d_vs_oracle(precision, phasing):
 
 
 @pytest.mark.parametrize("precision", ["single", "double"])
+@pytest.mark.parametrize("phasing", ["real", "complex"])
+@pytest.mark.parametrize("Nd,Kd,nreps", [((24, 20), (48, 40), 13), ((12, 10, 8), (24, 20, 16), 9), ((40,), (80,), 17)])
+def test_many_repetitions_vs_oracle(precision, phasing, Nd, Kd, nreps):
+    """Repetitions beyond the gather's staging chunk (8 per pass): chunk
+    boundaries, a partial last chunk, 1-D / 2-D / 3-D."""
+    om = np.random.default_rng(11).uniform(-np.pi, np.pi, (400, len(Nd)))
+    A, B = _pair(Nd, om, Jd=6, Kd=Kd, Ld=512, precision=precision, phasing=phasing)
+    x = trj.complex_gaussian(Nd + (nreps,), seed=12)
+    y = A.fft(x)
+    assert rel_l2(y, B.fft(x)) <= TOL[precision]
+    yb = trj.complex_gaussian((om.shape[0], nreps), seed=13)
+    assert rel_l2(A.adj(yb), B.adj(yb)) <= TOL[precision]
+    for r in (0, 7, 8, nreps - 1):  # each repetition on its own
+        assert rel_l2(y[:, r], B.fft(x[..., r])) <= TOL[precision]
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
 def test_operator_options_vs_oracle(precision):
     """ortho, adjoint_scalefactor, repetitions, order='C', real input."""
     Nd, Kd = (32, 32), (64, 64)
