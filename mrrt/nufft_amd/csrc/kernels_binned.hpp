@@ -592,21 +592,27 @@ __device__ __forceinline__ void cmadd(T& ar, T& ai, T cr, T ci, T vr, T vi) {
     madd<T, CK>(ar, ai, cr, ci, vr, vi);
 }
 
-template <typename T, int NDIM, bool CK, int JMAX, bool FIXED>
+// MULTI: a.rc repetitions' tiles are staged per pass and each point's
+// coefficients and derivatives are computed once for all of them.
+template <typename T, int NDIM, bool CK, int JMAX, bool FIXED, bool MULTI>
 __global__ __launch_bounds__(256) void grad_tiled_kernel(TiledArgs<T, CK> a, GradArgs<T> gr) {
     extern __shared__ __align__(16) unsigned char smem[];
-    cx<T>* tile = reinterpret_cast<cx<T>*>(smem);
+    cx<T>* tiles = reinterpret_cast<cx<T>*>(smem);
     const InterpArgs<T, CK>& ia = a.ia;
     const BinGeom& bg = a.bg;
     const AxisGeom ax[3] = {ia.ax[0], ia.ax[1], ia.ax[2]};
     const int T0 = bg.T[0], T1 = bg.T[1];
     const int tid = threadIdx.x;
     const int n_sub = a.bs.n_sub[0];
+    const int RC = MULTI ? a.rc : 1;
     for (int sid = blockIdx.x; sid < n_sub; sid += gridDim.x) {
     Sub s;
     load_sub(bg, a.bs, NDIM, sid, s);
-    for (int r = 0; r < ia.nreps; ++r) {
-        stage_tile<T, NDIM, CK>(a, s, ax, ia.grid + ((int64_t)s.frame * ia.nreps + r) * ia.kstride, tile);
+    for (int r0 = 0; r0 < ia.nreps; r0 += RC) {
+        const int rc = MULTI ? min(RC, ia.nreps - r0) : 1;
+        for (int rr = 0; rr < rc; ++rr)
+            stage_tile<T, NDIM, CK>(a, s, ax, ia.grid + ((int64_t)s.frame * ia.nreps + r0 + rr) * ia.kstride,
+                                    tiles + (int64_t)rr * bg.tile);
         __syncthreads();
         for (int i = s.start + tid; i < s.end; i += blockDim.x) {
             const int64_t m = a.bs.perm[i];
@@ -620,73 +626,81 @@ __global__ __launch_bounds__(256) void grad_tiled_kernel(TiledArgs<T, CK> a, Gra
                 else k0 = axis_coefs_d<T, JMAX, FIXED>(t, ia.L, ax[d], ia.h[d], cr[d], dr[d]);
                 l[d] = k0 - s.o[d];
             }
-            // S[0] = value, S[1 + d] = derivative along axis d
-            T Sr[4] = {0, 0, 0, 0}, Si[4] = {0, 0, 0, 0};
-            if constexpr (NDIM == 1) {
-#pragma unroll
-                for (int j1 = 0; j1 < JMAX; ++j1)
-                    if (FIXED || j1 < ax[0].J) {
-                        const cx<T> v = tile[l[0] + j1];
-                        cmadd<T, CK>(Sr[0], Si[0], cr[0][j1], ci[0][j1], v.x, v.y);
-                        cmadd<T, CK>(Sr[1], Si[1], dr[0][j1], di[0][j1], v.x, v.y);
-                    }
-            } else if constexpr (NDIM == 2) {
-#pragma unroll
-                for (int j2 = 0; j2 < JMAX; ++j2)
-                    if (FIXED || j2 < ax[1].J) {
-                        const cx<T>* row = tile + (l[1] + j2) * T0 + l[0];
-                        T ar = 0, ai = 0, br = 0, bi = 0;
-#pragma unroll
-                        for (int j1 = 0; j1 < JMAX; ++j1)
-                            if (FIXED || j1 < ax[0].J) {
-                                const cx<T> v = row[j1];
-                                cmadd<T, CK>(ar, ai, cr[0][j1], ci[0][j1], v.x, v.y);
-                                cmadd<T, CK>(br, bi, dr[0][j1], di[0][j1], v.x, v.y);
-                            }
-                        cmadd<T, CK>(Sr[0], Si[0], cr[1][j2], ci[1][j2], ar, ai);
-                        cmadd<T, CK>(Sr[1], Si[1], cr[1][j2], ci[1][j2], br, bi);
-                        cmadd<T, CK>(Sr[2], Si[2], dr[1][j2], di[1][j2], ar, ai);
-                    }
-            } else {
-#pragma unroll FIXED ? JMAX : 1
-                for (int j3 = 0; j3 < JMAX; ++j3)
-                    if (FIXED || j3 < ax[2].J) {
-                        T pr = 0, pi = 0, qr = 0, qi = 0, ur = 0, ui = 0;
-#pragma unroll
-                        for (int j2 = 0; j2 < JMAX; ++j2)
-                            if (FIXED || j2 < ax[1].J) {
-                                const cx<T>* row = tile + ((l[2] + j3) * T1 + (l[1] + j2)) * T0 + l[0];
-                                T ar = 0, ai = 0, br = 0, bi = 0;
-#pragma unroll
-                                for (int j1 = 0; j1 < JMAX; ++j1)
-                                    if (FIXED || j1 < ax[0].J) {
-                                        const cx<T> v = row[j1];
-                                        cmadd<T, CK>(ar, ai, cr[0][j1], ci[0][j1], v.x, v.y);
-                                        cmadd<T, CK>(br, bi, dr[0][j1], di[0][j1], v.x, v.y);
-                                    }
-                                cmadd<T, CK>(pr, pi, cr[1][j2], ci[1][j2], ar, ai);
-                                cmadd<T, CK>(qr, qi, cr[1][j2], ci[1][j2], br, bi);
-                                cmadd<T, CK>(ur, ui, dr[1][j2], di[1][j2], ar, ai);
-                            }
-                        cmadd<T, CK>(Sr[0], Si[0], cr[2][j3], ci[2][j3], pr, pi);
-                        cmadd<T, CK>(Sr[1], Si[1], cr[2][j3], ci[2][j3], qr, qi);
-                        cmadd<T, CK>(Sr[2], Si[2], cr[2][j3], ci[2][j3], ur, ui);
-                        cmadd<T, CK>(Sr[3], Si[3], dr[2][j3], di[2][j3], pr, pi);
-                    }
-            }
             cx<T> ph = mkcx<T>((T)1, (T)0);
             if (a.phs) ph = a.phs[i];
-            // y = S0 * ph
-            const T yr = Sr[0] * ph.x - Si[0] * ph.y;
-            const T yi = Sr[0] * ph.y + Si[0] * ph.x;
-            const cx<T> w = gr.w[(int64_t)r * ia.M + m];
+            T acc[3] = {0, 0, 0};  // sum over this pass's repetitions
+            for (int rr = 0; rr < rc; ++rr) {
+                const cx<T>* tile = tiles + (int64_t)rr * bg.tile;
+                // S[0] = value, S[1 + d] = derivative along axis d
+                T Sr[4] = {0, 0, 0, 0}, Si[4] = {0, 0, 0, 0};
+                if constexpr (NDIM == 1) {
+#pragma unroll
+                    for (int j1 = 0; j1 < JMAX; ++j1)
+                        if (FIXED || j1 < ax[0].J) {
+                            const cx<T> v = tile[l[0] + j1];
+                            cmadd<T, CK>(Sr[0], Si[0], cr[0][j1], ci[0][j1], v.x, v.y);
+                            cmadd<T, CK>(Sr[1], Si[1], dr[0][j1], di[0][j1], v.x, v.y);
+                        }
+                } else if constexpr (NDIM == 2) {
+#pragma unroll
+                    for (int j2 = 0; j2 < JMAX; ++j2)
+                        if (FIXED || j2 < ax[1].J) {
+                            const cx<T>* row = tile + (l[1] + j2) * T0 + l[0];
+                            T ar = 0, ai = 0, br = 0, bi = 0;
+#pragma unroll
+                            for (int j1 = 0; j1 < JMAX; ++j1)
+                                if (FIXED || j1 < ax[0].J) {
+                                    const cx<T> v = row[j1];
+                                    cmadd<T, CK>(ar, ai, cr[0][j1], ci[0][j1], v.x, v.y);
+                                    cmadd<T, CK>(br, bi, dr[0][j1], di[0][j1], v.x, v.y);
+                                }
+                            cmadd<T, CK>(Sr[0], Si[0], cr[1][j2], ci[1][j2], ar, ai);
+                            cmadd<T, CK>(Sr[1], Si[1], cr[1][j2], ci[1][j2], br, bi);
+                            cmadd<T, CK>(Sr[2], Si[2], dr[1][j2], di[1][j2], ar, ai);
+                        }
+                } else {
+#pragma unroll FIXED ? JMAX : 1
+                    for (int j3 = 0; j3 < JMAX; ++j3)
+                        if (FIXED || j3 < ax[2].J) {
+                            T pr = 0, pi = 0, qr = 0, qi = 0, ur = 0, ui = 0;
+#pragma unroll
+                            for (int j2 = 0; j2 < JMAX; ++j2)
+                                if (FIXED || j2 < ax[1].J) {
+                                    const cx<T>* row = tile + ((l[2] + j3) * T1 + (l[1] + j2)) * T0 + l[0];
+                                    T ar = 0, ai = 0, br = 0, bi = 0;
+#pragma unroll
+                                    for (int j1 = 0; j1 < JMAX; ++j1)
+                                        if (FIXED || j1 < ax[0].J) {
+                                            const cx<T> v = row[j1];
+                                            cmadd<T, CK>(ar, ai, cr[0][j1], ci[0][j1], v.x, v.y);
+                                            cmadd<T, CK>(br, bi, dr[0][j1], di[0][j1], v.x, v.y);
+                                        }
+                                    cmadd<T, CK>(pr, pi, cr[1][j2], ci[1][j2], ar, ai);
+                                    cmadd<T, CK>(qr, qi, cr[1][j2], ci[1][j2], br, bi);
+                                    cmadd<T, CK>(ur, ui, dr[1][j2], di[1][j2], ar, ai);
+                                }
+                            cmadd<T, CK>(Sr[0], Si[0], cr[2][j3], ci[2][j3], pr, pi);
+                            cmadd<T, CK>(Sr[1], Si[1], cr[2][j3], ci[2][j3], qr, qi);
+                            cmadd<T, CK>(Sr[2], Si[2], cr[2][j3], ci[2][j3], ur, ui);
+                            cmadd<T, CK>(Sr[3], Si[3], dr[2][j3], di[2][j3], pr, pi);
+                        }
+                }
+                // y = S0 * ph
+                const T yr = Sr[0] * ph.x - Si[0] * ph.y;
+                const T yi = Sr[0] * ph.y + Si[0] * ph.x;
+                const cx<T> w = gr.w[(int64_t)(r0 + rr) * ia.M + m];
+#pragma unroll
+                for (int d = 0; d < NDIM; ++d) {
+                    const T er = (Sr[1 + d] * ph.x - Si[1 + d] * ph.y) * gr.dtdw[d] - gr.shift[d] * yi;
+                    const T ei = (Sr[1 + d] * ph.y + Si[1 + d] * ph.x) * gr.dtdw[d] + gr.shift[d] * yr;
+                    const T v = w.x * er + w.y * ei;
+                    acc[d] = (rr == 0) ? v : acc[d] + v;
+                }
+            }
 #pragma unroll
             for (int d = 0; d < NDIM; ++d) {
-                const T er = (Sr[1 + d] * ph.x - Si[1 + d] * ph.y) * gr.dtdw[d] - gr.shift[d] * yi;
-                const T ei = (Sr[1 + d] * ph.y + Si[1 + d] * ph.x) * gr.dtdw[d] + gr.shift[d] * yr;
-                const T v = w.x * er + w.y * ei;
                 T* o = gr.grad + m * NDIM + d;
-                *o = (r == 0) ? v : *o + v;
+                *o = (r0 == 0) ? acc[d] : *o + acc[d];
             }
         }
         __syncthreads();

@@ -1035,7 +1035,12 @@ int run_tiled_any(mi355nufft_plan* p, bool adjoint, const void* samples_in, void
 template <typename T, bool CK, int NDIM, int JMAX, bool FIXED>
 void launch_grad(const TiledArgs<T, CK>& a, const GradArgs<T>& g, unsigned nsub, unsigned threads, size_t lds,
                  hipStream_t s) {
-    hipLaunchKernelGGL((grad_tiled_kernel<T, NDIM, CK, JMAX, FIXED>), dim3(nsub), dim3(threads), lds, s, a, g);
+    if (a.rc > 1)
+        hipLaunchKernelGGL((grad_tiled_kernel<T, NDIM, CK, JMAX, FIXED, true>), dim3(nsub), dim3(threads), lds, s, a,
+                           g);
+    else
+        hipLaunchKernelGGL((grad_tiled_kernel<T, NDIM, CK, JMAX, FIXED, false>), dim3(nsub), dim3(threads), lds, s,
+                           a, g);
 }
 
 template <typename T, bool CK>
@@ -1054,7 +1059,6 @@ int run_grad(mi355nufft_plan* p, const void* grid, const void* w, void* grad, in
     a.scale_ortho = (T)p->scale_ortho;
     a.tms = (const T*)p->d_tms;
     a.phs = p->has_phase ? (const cx<T>*)p->d_phs : nullptr;
-    a.rc = 1;
     GradArgs<T> g;
     g.w = (const cx<T>*)w;
     g.grad = (T*)grad;
@@ -1063,7 +1067,14 @@ int run_grad(mi355nufft_plan* p, const void* grid, const void* w, void* grad, in
         g.shift[d] = (T)p->shift[d];
     }
     const unsigned nsub = tiled_grid(p);
-    const size_t lds = gather_tile_bytes(p);
+    {  // repetitions staged per pass, as in the gather
+        const size_t chunk_lds = p->lds_limit < kRepChunkLds ? p->lds_limit : kRepChunkLds;
+        int rc = (int)(chunk_lds / gather_tile_bytes(p));
+        rc = rc < kMaxRepChunk ? rc : kMaxRepChunk;
+        rc = rc < nreps ? rc : nreps;
+        a.rc = rc > 1 ? rc : 1;
+    }
+    const size_t lds = gather_tile_bytes(p) * (size_t)a.rc;
     const int jm = jmax_of(p->ndim, J);
     const int jf = fixed_j(p->ndim, J);
 #define MI355_GFCASE(ND, JF)                                           \

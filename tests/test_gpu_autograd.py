@@ -197,3 +197,26 @@ def test_points_update_in_place_is_seen():
     B = O.NufftOracle(Nd, om * 0.5, Jd=6, Kd=Kd, Ld=1024, precision="double")
     assert rel_l2(y2.cpu().numpy(), B.fft(x.cpu().numpy())) <= 1e-12
     assert rel_l2(y1.cpu().numpy(), y2.cpu().numpy()) > 1e-3
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_point_gradient_is_sum_over_coils(ndim):
+    """The trajectory gradient of an 11-coil batch (repetitions staged 8 per
+    pass, then 3) equals the sum of the 11 single-coil gradients."""
+    if ndim == 2:
+        Nd, Kd = (16, 12), (32, 24)
+        om = trj.golden_angle_2d(10, 32) * 0.9
+    else:
+        Nd, Kd = (8, 6, 8), (16, 12, 16)
+        om = trj.kooshball_3d(12, 16, seed=5) * 0.9
+    op = NufftBase(Nd, om, Jd=6, Kd=Kd, Ld=1024, precision="double", device=dev())
+    C = 11
+    x = torch.from_numpy(trj.complex_gaussian((C,) + Nd, seed=6)).to(dev())
+    y_t = torch.from_numpy(trj.complex_gaussian((C, om.shape[0]), seed=7)).to(dev())
+    omega = torch.from_numpy(om).to(dev()).requires_grad_(True)
+    (gw_all,) = torch.autograd.grad(_loss_fn(op, y_t)(x, omega), (omega,))
+    gw_sum = torch.zeros_like(gw_all)
+    for c in range(C):
+        (g,) = torch.autograd.grad(_loss_fn(op, y_t[c:c + 1])(x[c:c + 1], omega), (omega,))
+        gw_sum += g
+    assert rel_l2(gw_all.cpu().numpy(), gw_sum.cpu().numpy()) <= 1e-12
