@@ -78,13 +78,17 @@ def interp_bytes(M, ndim, nk, nreps, real_size):
 
 
 class GpuRun:
-    def __init__(self, wl):
+    def __init__(self, wl, mode="table"):
         import torch
         from mrrt.nufft_amd import NufftBase
         self.torch = torch
         self.dev = torch.device("cuda", 0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         self.A = NufftBase(wl["Nd"], wl["om"], Jd=wl["Jd"], Kd=wl["Kd"], Ld=wl["Ld"], precision=wl["precision"],
-                           device=self.dev)
+                           mode=mode, device=self.dev)
+        torch.cuda.synchronize()
+        self.setup_s = time.perf_counter() - t0
         cdt = torch.complex64 if wl["precision"] == "single" else torch.complex128
         nd = len(wl["Nd"])
         xr = torch.from_numpy(np.ascontiguousarray(wl["x"])).to(self.dev).to(cdt)
@@ -265,6 +269,27 @@ def run_c5(steps=5):
             "c5_autograd_overhead": round(t_ad / t_pl, 3)}
 
 
+def run_sparse(wl, tag, steps, warmup, fft):
+    """Forward + adjoint with the precomputed sparse matrix (mode='sparse'):
+    pair rate, stage times, set-up time and the matrix's device footprint
+    (rows + columns: values, indices)."""
+    r = GpuRun(wl, mode="sparse")
+    r.A.set_fft(fft)
+    t = r.time(r.pair, steps, warmup)
+    st = r.stage_times(max(5, steps // 2), min(warmup, 3))
+    info = r.A.plan_info()
+    vsize = 4 if wl["precision"] == "single" else 8
+    out = {
+        tag + "_sparse_pairs_per_s": round(1000.0 / t, 2),
+        tag + "_sparse_stages_ms": {k: round(v, 4) for k, v in st.items()},
+        tag + "_sparse_setup_s": round(r.setup_s, 3),
+        tag + "_sparse_nnz": info["sparse_nnz"],
+        tag + "_sparse_matrix_MB": round(2 * info["sparse_nnz"] * (vsize + 4) / 1e6, 1),
+    }
+    del r
+    return out
+
+
 def _time_fn(fn, steps, warmup):
     import torch
     for _ in range(warmup):
@@ -396,6 +421,10 @@ def main():
         t2 = r2.time(r2.pair, max(5, args.steps // 2), 3)
         extra["c2_complex128_pairs_per_s"] = round(1000.0 / t2, 2)
         del r2
+        # sparse-matrix mode (mode='sparse') on the same workload
+        print("bench: sparse mode", file=sys.stderr, flush=True)
+        extra.update(run_sparse(c2_workload("single"), "c2", args.steps, args.warmup, fft_choice))
+        torch.cuda.empty_cache()
         r3 = GpuRun(c3_workload())
         t3 = r3.time(r3.pair, 5, 1)
         st3 = r3.stage_times(5, 1)
@@ -405,6 +434,8 @@ def main():
         extra["c3_interp_GBps"] = round(b3 / (st3["interp"] * 1e-3) / 1e9, 1)
         extra["c3_spread_GBps"] = round(b3 / (st3["spread"] * 1e-3) / 1e9, 1)
         del r3
+        torch.cuda.empty_cache()
+        extra.update(run_sparse(c3_workload(), "c3", 5, 1, "auto"))
         torch.cuda.empty_cache()
         print("bench: frames batch (24 x 32)", file=sys.stderr, flush=True)
         extra.update(run_c4())

@@ -163,11 +163,12 @@ int mi355nufft_read_timing(mi355nufft_plan *plan, int32_t which, int64_t *launch
                            double *total_ms);
 
 /*
- * Plan description: info[0..14] = ndim, M, bin size (3), points per
+ * Plan description: info[0..17] = ndim, M, bin size (3), points per
  * subproblem, number of bins, tile points, tiled kernels active (0/1),
  * subproblem slots, method, LDS budget in bytes, number of frames,
  * two-phase scatter active (0/1), threads per tiled workgroup, fused pruned
- * FFT active (0/1).
+ * FFT active (0/1), sparse-matrix entries (-1 in table mode), sparse-matrix
+ * column pieces.
  */
 int mi355nufft_plan_info(const mi355nufft_plan *plan, int64_t *info, int32_t n);
 
@@ -214,13 +215,15 @@ int mi355nufft_fft(mi355nufft_plan *plan, void *grid, int32_t nreps,
 
 /* Forward interpolation: grid * phase_before (/ sqrt(prod Kd) if ortho),
  * table1 gather, * phase_after / phase_shift: _nufft.py:1370-1391 with
- * _nufft_table_interp (:998-1097).  The grid is left unchanged. */
+ * _nufft_table_interp (:998-1097).  The tiled kernels leave the grid
+ * unchanged; the global-memory kernels and sparse mode multiply it in place
+ * by phase_before (and 1/sqrt(prod Kd)), as the reference does at :1371. */
 int mi355nufft_interp(mi355nufft_plan *plan, const void *grid, void *y,
                       int32_t nreps);
 
 /* Adjoint gridding: y * conj(phase_after), table1 scatter onto a zeroed
  * grid, * conj(phase_before): _nufft.py:1495-1520 with _nufft_table_adj
- * (:1101-1191). */
+ * (:1101-1191).  In sparse mode the interpolation is P^H y (:1505). */
 int mi355nufft_spread(mi355nufft_plan *plan, const void *y, void *grid,
                       int32_t nreps);
 
@@ -272,6 +275,42 @@ int mi355nufft_adjoint_image(mi355nufft_plan *plan, void *grid, void *x,
  * No counterpart in the reference (which calls the array module's fftn).
  */
 int mi355nufft_set_fft(mi355nufft_plan *plan, int method);
+
+/* ---- sparse mode ---- */
+
+/*
+ * Sparse-matrix mode: builds the interpolation matrix P (M x prod Kd) of
+ * NufftBase._init_sparsemat (mrrt/nufft/_nufft.py:751-877) on the device from
+ * its per-axis factors, for the points of the last mi355nufft_set_points.
+ * Until mi355nufft_clear_sparse or the next mi355nufft_set_points, the
+ * plan's forward / interp use x = P xk (:1383-1384) and adjoint / spread use
+ * xk = P^H y (:1505) instead of the table1 kernels; phase_before,
+ * phase_after, ortho and the FFTs are unchanged.
+ *   coef[d]  device (J_d, M) factors ud[d] of :783-802: float64 c (phasing
+ *            0) or complex128 exp(i gam (N_d-1)/2 arg) * c (phasing 1)
+ *   kidx[d]  device (J_d, M) int32 mod(koff + 1..J_d, K_d) (:788-791)
+ *   phase    device complex128 (M) exp(i omega . n_shift) of :833-838
+ *            (phasing 1 with a nonzero n_shift), or NULL
+ * Entry (m, kk) = prod_d ud[d] in float64 / complex128 (conjugated for
+ * phasing 1, times phase), rounded to the plan's precision (:854-858).
+ * The matrix is held twice (rows for the forward, columns for the
+ * adjoint), 2 * M * prod(Jd) values and indices; the adjoint writes every
+ * grid point once, without atomics, so it is bit-reproducible.  Needs
+ * M * prod(Jd) < 2^31 and one frame.
+ */
+int mi355nufft_set_sparse(mi355nufft_plan *plan, const void *const *coef,
+                          const int32_t *const *kidx, const void *phase);
+
+/* Frees the sparse matrix; the plan returns to the table1 kernels. */
+int mi355nufft_clear_sparse(mi355nufft_plan *plan);
+
+/*
+ * The bare products of the sparse matrix, without any phase or scale:
+ * adjoint 0: out (nreps, M) = P in (nreps, prod Kd)   (obj.p * xk, :1384)
+ * adjoint 1: out (nreps, prod Kd) = P^H in (nreps, M) (obj.p.H * xk, :1505)
+ */
+int mi355nufft_sparse_apply(mi355nufft_plan *plan, int adjoint, const void *in,
+                            void *out, int32_t nreps);
 
 /* ---- raw table1 kernels (no plan) ---- */
 

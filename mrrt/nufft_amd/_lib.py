@@ -60,6 +60,9 @@ _SIGNATURES = {
     "mi355nufft_forward_grid": (c_int, [c_vp, c_vp, c_i64p, c_vp, c_i32]),
     "mi355nufft_adjoint_image": (c_int, [c_vp, c_vp, c_vp, c_i64p, c_i32]),
     "mi355nufft_set_fft": (c_int, [c_vp, c_int]),
+    "mi355nufft_set_sparse": (c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "mi355nufft_clear_sparse": (c_int, [c_vp]),
+    "mi355nufft_sparse_apply": (c_int, [c_vp, c_int, c_vp, c_vp, c_i32]),
     "mi355nufft_toeplitz_create": (c_int, [ctypes.POINTER(c_vp), c_int, c_i64p, c_int, c_vp]),
     "mi355nufft_toeplitz_destroy": (c_int, [c_vp]),
     "mi355nufft_toeplitz_set_stream": (c_int, [c_vp, c_vp]),

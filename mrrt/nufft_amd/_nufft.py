@@ -21,7 +21,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._kernel_math import beatty_alpha, build_table, deapod_factors
+from ._kernel_math import beatty_alpha, build_table, deapod_factors, kaiser_bessel
 
 __all__ = ["NufftBase", "nufft_forward", "nufft_adj"]
 
@@ -59,12 +59,13 @@ def _f_reshape(t, shape):
 
 
 class NufftBase(object):
-    """Table-mode ("table1") NUFFT operator on one MI355X.
+    """Table-mode ("table1") or sparse-matrix NUFFT operator on one MI355X.
 
     Parameters follow the reference (mrrt/nufft/_nufft.py:122-233).  ``om``
-    is accepted as an alias of ``omega``.  ``mode`` must contain "table"
-    (``'table'`` or ``'table1'``); sparse-matrix mode is not provided.
-    ``device`` selects the GPU (default: the current torch device).
+    is accepted as an alias of ``omega``.  ``mode`` is ``'sparse'`` (the
+    precomputed interpolation matrix of :751-877, held on the device) or
+    contains "table" (``'table'`` or ``'table1'``).  ``device`` selects the
+    GPU (default: the current torch device).
     """
 
     def __init__(self, Nd, omega=None, Jd=4, Kd=None, precision="single", mode="table", Ld=1024,
@@ -95,11 +96,13 @@ class NufftBase(object):
             raise NotImplementedError("dimensions > 3 not implemented")
         self.ortho = ortho
         self.scale_ortho = sqrt(float(np.prod(self._Kd))) if ortho else 1
-        if not ("table" in mode):
-            if mode == "exact":
-                raise ValueError("mode exact not implemented")
-            if mode == "sparse":
-                raise NotImplementedError("sparse-matrix mode is not provided; use mode='table'")
+        if mode == "sparse":                            # _nufft.py:321-350
+            self._sparse = True
+        elif "table" in mode:
+            self._sparse = False
+        elif mode == "exact":
+            raise ValueError("mode exact not implemented")
+        else:
             raise ValueError("Invalid NUFFT mode: {}".format(mode))
         self._mode = mode
         self._Ld = int(Ld) if Ld is not None else 2 ** 10
@@ -174,6 +177,9 @@ class NufftBase(object):
     Ld = property(lambda self: self._Ld)
     mode = property(lambda self: self._mode)
     phasing = property(lambda self: self._phasing)
+    # _nufft.py:510-513; the matrix itself lives on the device (rows for the
+    # forward, columns for the adjoint), not as a host scipy matrix
+    sparse_format = property(lambda self: "CSC" if self._sparse else None)
     precision = property(lambda self: self._precision)
     n_shift = property(lambda self: self._n_shift)
 
@@ -224,7 +230,9 @@ class NufftBase(object):
         if self._phasing != "real":
             return None
         shift = np.asarray([s - m for s, m in zip(self._n_shift, self.n_mid)], dtype=self._real_dtype)
-        ph = np.exp(1j * np.dot(self._omega_host, shift))
+        # sparse mode recomputes it from the cast omega (_nufft.py:768-769 after :315)
+        om = self._omega_rdt.cpu().numpy() if self._sparse else self._omega_host
+        ph = np.exp(1j * np.dot(om, shift))
         return torch.from_numpy(ph.astype(self._cplx_dtype, copy=False)).to(self.device)
 
     # ------------------------------------------------------------------
@@ -273,11 +281,16 @@ class NufftBase(object):
         self._omega_np = omega_np
         self._omega_rdt = omega_t.to(self._trdt)
         self._points_version = getattr(self, "_points_version", 0) + 1
+        if self._sparse and frame_offsets is not None:
+            raise ValueError("sparse mode takes one frame")
+        # sparse mode: phase_after from the cast omega (_nufft.py:768-769);
+        # the complex kernel's phase_shift is folded into the matrix (:833-838)
+        om = self._omega_rdt if self._sparse else omega_t
         if self._phasing == "real":
             shift = [float(np.asarray(s - m, dtype=self._real_dtype)) for s, m in zip(self._n_shift, self.n_mid)]
             phase_mode = 1
-            dot_single = int(omega_t.dtype == torch.float32 and self._precision == "single")
-        elif any(s != 0 for s in self._n_shift):
+            dot_single = int(om.dtype == torch.float32 and self._precision == "single")
+        elif any(s != 0 for s in self._n_shift) and not self._sparse:
             shift = list(self._n_shift)
             phase_mode = 2
             dot_single = 0
@@ -285,7 +298,6 @@ class NufftBase(object):
             shift = [0.0] * self.ndim
             phase_mode = 0
             dot_single = 0
-        om = omega_t
         self._sync_stream()
         if frame_offsets is None:
             offs, nframes = None, 1
@@ -297,6 +309,48 @@ class NufftBase(object):
             om.stride(0), om.stride(1), _lib.dblarr(shift + [0.0] * (3 - self.ndim)), phase_mode, dot_single,
             offs, nframes))
         self.nframes = nframes
+        if self._sparse:
+            self._init_sparsemat()
+
+    def _init_sparsemat(self):
+        """Interpolation matrix of mode='sparse' (_init_sparsemat,
+        _nufft.py:751-877).
+
+        The per-axis factors ud[d] and grid indices kd[d] are computed here as
+        in the reference, from omega cast to the real dtype (:772-802 with
+        _nufft_coef / _nufft_offset, _utils.py:20-86); the device forms the
+        entries prod_d ud[d] in float64 / complex128, conjugated for the
+        complex kernel and times exp(i omega . n_shift) (:815-838), rounded to
+        the operator's precision (``mi355nufft_set_sparse``).
+        """
+        om = self._omega_rdt.cpu().numpy()
+        cplx = self._phasing == "complex"
+        coefs, kidx = [], []
+        for d in range(self.ndim):
+            N, J, K = self._Nd[d], self._Jd[d], self._Kd[d]
+            o = om[:, d]
+            gam = 2 * np.pi / K
+            koff = np.floor(o / gam - J / 2.0)                          # _utils.py:40-41
+            dk = o / gam - koff                                         # _utils.py:74-75
+            arg = -np.arange(1, J + 1)[:, None] + dk[None, :]           # _utils.py:78, (J, M)
+            c = kaiser_bessel(arg, J, self.alpha[d])
+            kd = np.mod(np.arange(1, J + 1)[:, None] + koff[None, :], K)   # :791
+            if cplx:
+                phase_scale = 1j * gam * (N - 1) / 2.0                  # :793-796
+                ud = np.exp(phase_scale * arg) * c
+            else:
+                ud = c
+            ud = np.ascontiguousarray(ud, dtype=np.complex128 if cplx else np.float64)
+            coefs.append(torch.from_numpy(ud).to(self.device))
+            kidx.append(torch.from_numpy(np.ascontiguousarray(kd.astype(np.int32))).to(self.device))
+        phase = None
+        if cplx and any(s != 0 for s in self._n_shift):                 # :833-838
+            phase = torch.from_numpy(np.exp(1j * np.dot(om, np.asarray(self._n_shift)))).to(self.device)
+        cp = (ctypes.c_void_p * self.ndim)(*[t.data_ptr() for t in coefs])
+        kp = (ctypes.c_void_p * self.ndim)(*[t.data_ptr() for t in kidx])
+        self._sync_stream()
+        _lib.check(self._lib.mi355nufft_set_sparse(self._plan, cp, kp,
+                                                   None if phase is None else ctypes.c_void_p(phase.data_ptr())))
 
     _METHODS = {"auto": 0, "gm": 1, "tiled": 2, "tiled_atomic": 3, "tiled_two_phase": 4}
 
@@ -320,12 +374,13 @@ class NufftBase(object):
 
     def plan_info(self):
         """dict with the binning state of the plan (see mi355nufft_plan_info)."""
-        buf = (ctypes.c_int64 * 16)()
-        _lib.check(self._lib.mi355nufft_plan_info(self._plan, buf, 16))
+        buf = (ctypes.c_int64 * 18)()
+        _lib.check(self._lib.mi355nufft_plan_info(self._plan, buf, 18))
         v = list(buf)
         return dict(ndim=v[0], M=v[1], bin_size=tuple(v[2:5]), max_points=v[5], nbins=v[6], tile=v[7],
                     tiled=bool(v[8]), subproblem_slots=v[9], method=v[10], lds_limit=v[11], nframes=v[12],
-                    two_phase=bool(v[13]), threads=v[14], fused_fft=bool(v[15]))
+                    two_phase=bool(v[13]), threads=v[14], fused_fft=bool(v[15]),
+                    sparse_nnz=v[16] if v[16] >= 0 else None, sparse_pieces=v[17])
 
     def set_fft(self, method="auto"):
         """FFT of the transforms: "auto" (pruned passes fused with the
@@ -489,6 +544,18 @@ def _raw_table1(obj, fwd, data, nreps):
     return out
 
 
+def _sparse_apply(obj, adjoint, data, nreps):
+    """Bare sparse-matrix product: P data (forward) or P^H data (adjoint)."""
+    if adjoint:
+        out = torch.empty((nreps, int(np.prod(obj.Kd))), dtype=obj._tcdt, device=obj.device)
+    else:
+        out = torch.empty((nreps, obj.M), dtype=obj._tcdt, device=obj.device)
+    obj._sync_stream()
+    _lib.check(obj._lib.mi355nufft_sparse_apply(obj._plan, 1 if adjoint else 0, ctypes.c_void_p(data.data_ptr()),
+                                                ctypes.c_void_p(out.data_ptr()), nreps))
+    return out
+
+
 def _phase_shift_dev(obj):
     """Complex-kernel phase_shift exp(i omega_rdt . n_shift), complex128 (_nufft.py:898-901)."""
     if obj._phasing != "complex" or not any(s != 0 for s in obj.n_shift):
@@ -512,6 +579,8 @@ def nufft_forward(obj, x, grid_only=False):
             raise ValueError("Input signal has the wrong size.")
         nreps = t.numel() // nk
         g = _f_reshape(t, (nk, nreps)).t().contiguous()
+        if obj._sparse:                       # x = obj.p * xk (:1384), phase_shift inside p
+            return _to_output(_sparse_apply(obj, False, g, nreps).t(), was_numpy)
         y = _raw_table1(obj, True, g, nreps)
         ph = _phase_shift_dev(obj)
         if ph is not None:
@@ -546,10 +615,13 @@ def nufft_adj(obj, xk, return_psf=False, grid_only=False):
     if grid_only or return_psf:
         if obj._phasing == "real" and not return_psf:
             y = y * torch.conj(obj.phase_after)[None, :]
-        ph = _phase_shift_dev(obj)
-        if ph is not None:
-            y = (y.to(torch.complex128) * torch.conj(ph)[None, :]).to(obj._tcdt)
-        g = _raw_table1(obj, False, y.contiguous(), nreps)
+        if obj._sparse:                       # xk = obj.p.H * xk (:1505), phase_shift inside p
+            g = _sparse_apply(obj, True, y.contiguous(), nreps)
+        else:
+            ph = _phase_shift_dev(obj)
+            if ph is not None:
+                y = (y.to(torch.complex128) * torch.conj(ph)[None, :]).to(obj._tcdt)
+            g = _raw_table1(obj, False, y.contiguous(), nreps)
         if return_psf:
             g0 = g[0].reshape(tuple(reversed(obj.Kd))).permute(*reversed(range(obj.ndim)))
             return _to_output(g0, was_numpy)

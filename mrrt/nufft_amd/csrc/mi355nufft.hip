@@ -5,6 +5,7 @@
 // enqueued on the plan's stream.
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 #include <rocfft/rocfft.h>
 
 #include <cmath>
@@ -19,6 +20,7 @@
 #include "kernels_interp.hpp"
 #include "kernels_binned.hpp"
 #include "kernels_fft.hpp"
+#include "kernels_sparse.hpp"
 
 using namespace mi355;
 
@@ -60,6 +62,24 @@ struct FftPlans {
     int nreps;
     rocfft_plan fwd;
     rocfft_plan bwd;
+};
+
+// Sparse-mode matrix (kernels_sparse.hpp): row form for the forward, column
+// pieces for the adjoint.
+struct SparseState {
+    int on = 0;
+    int cvals = 0;  // complex values (complex kernel)
+    int nnzr = 0;   // entries per row, prod J
+    int64_t M = 0, nnz = 0, npieces = 0, nsplit = 0, nslots = 0;
+    void* val = nullptr;       // (nnzr, M) real or complex, sorted rows
+    int32_t* col = nullptr;    // (nnzr, M) grid index
+    int32_t* perm = nullptr;   // sorted row -> sample
+    void* tval = nullptr;      // (nnz) values in column order
+    int32_t* trow = nullptr;   // (nnz) sample of each column entry
+    SpPiece* pieces = nullptr;
+    SpSplit* splits = nullptr;
+    void* part = nullptr;      // partial sums of split columns (nslots x nreps)
+    size_t part_bytes = 0;
 };
 
 }  // namespace
@@ -133,6 +153,8 @@ struct mi355nufft_plan {
     int tev_used[2] = {0, 0};
     double tev_ms[2] = {0, 0};       // folded totals of earlier pairs
     int64_t tev_n[2] = {0, 0};
+
+    SparseState sp;
 
     std::vector<FftPlans> ffts;
     rocfft_execution_info fft_info = nullptr;
@@ -670,6 +692,203 @@ int adjoint_image_impl(mi355nufft_plan* p, void* grid, void* x, const int64_t* x
     return fft_rows<T>(f, true, nullptr, x, xs, grid, nimg, (T)adjoint_image_scale(p), o);
 }
 
+// ---------------------------------------------------------------- sparse mode
+void sparse_free(mi355nufft_plan* p) {
+    SparseState& s = p->sp;
+    void* bufs[] = {s.val, s.col, s.perm, s.tval, s.trow, s.pieces, s.splits, s.part};
+    bool any = false;
+    for (void* b : bufs) any = any || b;
+    if (any) (void)hipStreamSynchronize(p->stream);
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    s = SparseState();
+}
+
+// Device temporaries of one call, freed after the stream drains on every
+// exit path.
+struct DevTemps {
+    hipStream_t s;
+    std::vector<void*> v;
+    explicit DevTemps(hipStream_t st) : s(st) {}
+    void* get(size_t bytes) {
+        void* q = nullptr;
+        if (hipMalloc(&q, bytes ? bytes : 1) != hipSuccess) return nullptr;
+        v.push_back(q);
+        return q;
+    }
+    ~DevTemps() {
+        if (v.empty()) return;
+        (void)hipStreamSynchronize(s);
+        for (void* q : v) (void)hipFree(q);
+    }
+};
+
+template <typename T, bool CV>
+int sparse_build(mi355nufft_plan* p, const void* const* coef, const int32_t* const* kidx, const void* phase) {
+    using V = tab<T, CV>;
+    SparseState& s = p->sp;
+    const int64_t M = p->M, nk = p->nk;
+    int nnzr = 1;
+    for (int d = 0; d < p->ndim; ++d) nnzr *= p->J[d];
+    const int64_t nnz = M * nnzr;
+    const int64_t i32max = ((int64_t)1 << 31) - 1;
+    if (nnz > i32max || nk + nnz / kSpPiece + 2 > i32max)
+        return fail(MI355NUFFT_EINVAL, "sparse matrix of %lld entries on %lld grid points exceeds the 2^31 index range",
+                    (long long)nnz, (long long)nk);
+    if (p->nframes > 1) return fail(MI355NUFFT_EINVAL, "sparse mode takes one frame");
+    s.cvals = CV;
+    s.nnzr = nnzr;
+    s.M = M;
+    s.nnz = nnz;
+    auto alloc = [&](void** q, size_t bytes) { return hipMalloc(q, bytes ? bytes : 1) == hipSuccess; };
+    if (!alloc(&s.val, (size_t)nnz * sizeof(V)) || !alloc((void**)&s.col, (size_t)nnz * 4) ||
+        !alloc((void**)&s.perm, (size_t)(M ? M : 1) * 4) || !alloc(&s.tval, (size_t)nnz * sizeof(V)) ||
+        !alloc((void**)&s.trow, (size_t)nnz * 4)) {
+        sparse_free(p);
+        return fail(MI355NUFFT_ENOMEM, "sparse matrix of %lld entries", (long long)nnz);
+    }
+    DevTemps tmp(p->stream);
+    uint32_t* key[2] = {(uint32_t*)tmp.get((size_t)nnz * 4), (uint32_t*)tmp.get((size_t)nnz * 4)};
+    uint32_t* pay[2] = {(uint32_t*)tmp.get((size_t)nnz * 4), (uint32_t*)tmp.get((size_t)nnz * 4)};
+    int32_t* ptr = (int32_t*)tmp.get((size_t)(nk + 1) * 4);
+    int32_t* cnt = (int32_t*)tmp.get((size_t)(nk + 1) * 4 * 3);
+    int32_t* off = (int32_t*)tmp.get((size_t)(nk + 1) * 4 * 3);
+    if (!key[0] || !key[1] || !pay[0] || !pay[1] || !ptr || !cnt || !off) {
+        sparse_free(p);
+        return fail(MI355NUFFT_ENOMEM, "sparse matrix build temporaries");
+    }
+    // row order: the bin order of the tiled kernels when present
+    if (M > 0) {
+        if (p->binned) {
+            HIP_TRY(hipMemcpyAsync(s.perm, p->d_bin_i + 2 * p->bin_m_cap, (size_t)M * 4, hipMemcpyDeviceToDevice,
+                                   p->stream));
+        } else {
+            hipLaunchKernelGGL(sp_iota_kernel, dim3(nblocks(M)), dim3(kBlock), 0, p->stream, s.perm, M);
+            LAUNCH_CHECK();
+        }
+        const int J1 = p->ndim > 1 ? p->J[1] : 1, J2 = p->ndim > 2 ? p->J[2] : 1;
+        hipLaunchKernelGGL((sp_fill_rows_kernel<T, CV>), dim3(nblocks(M)), dim3(kBlock), 0, p->stream, coef[0],
+                           p->ndim > 1 ? coef[1] : nullptr, p->ndim > 2 ? coef[2] : nullptr, kidx[0],
+                           p->ndim > 1 ? kidx[1] : nullptr, p->ndim > 2 ? kidx[2] : nullptr, (const double2*)phase,
+                           s.perm, M, p->ndim, p->J[0], J1, J2, (int)p->K[0], (int)(p->ndim > 1 ? p->K[1] : 1),
+                           (V*)s.val, s.col, key[0], pay[0]);
+        LAUNCH_CHECK();
+        int bits = 1;
+        while (((int64_t)1 << bits) < nk) ++bits;
+        size_t tb = 0;
+        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key[0], key[1], pay[0], pay[1], (int)nnz, 0, bits,
+                                                   p->stream));
+        void* ctmp = tmp.get(tb);
+        if (!ctmp) {
+            sparse_free(p);
+            return fail(MI355NUFFT_ENOMEM, "sparse matrix sort storage");
+        }
+        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(ctmp, tb, key[0], key[1], pay[0], pay[1], (int)nnz, 0, bits,
+                                                   p->stream));
+        hipLaunchKernelGGL((sp_fill_cols_kernel<T, CV>), dim3(nblocks(nnz)), dim3(kBlock), 0, p->stream, pay[1],
+                           nnz, nnzr, M, s.perm, (const V*)s.val, (V*)s.tval, s.trow);
+        LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(sp_col_ptr_kernel, dim3(nblocks(nk + 1)), dim3(kBlock), 0, p->stream, key[1], nnz, nk, ptr);
+    LAUNCH_CHECK();
+    int32_t* npc = cnt;
+    int32_t* nsl = cnt + (nk + 1);
+    int32_t* nsp = cnt + 2 * (nk + 1);
+    hipLaunchKernelGGL(sp_piece_count_kernel, dim3(nblocks(nk + 1)), dim3(kBlock), 0, p->stream, ptr, nk, npc, nsl,
+                       nsp);
+    LAUNCH_CHECK();
+    size_t sb = 0;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, sb, npc, off, (int)(nk + 1), p->stream));
+    void* stmp = tmp.get(sb);
+    if (!stmp) {
+        sparse_free(p);
+        return fail(MI355NUFFT_ENOMEM, "sparse matrix scan storage");
+    }
+    for (int q = 0; q < 3; ++q)
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(stmp, sb, cnt + q * (nk + 1), off + q * (nk + 1), (int)(nk + 1),
+                                                 p->stream));
+    int32_t tot[3];
+    for (int q = 0; q < 3; ++q)
+        HIP_TRY(hipMemcpyAsync(&tot[q], off + q * (nk + 1) + nk, 4, hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    s.npieces = tot[0];
+    s.nslots = tot[1];
+    s.nsplit = tot[2];
+    if (!alloc((void**)&s.pieces, (size_t)s.npieces * sizeof(SpPiece)) ||
+        !alloc((void**)&s.splits, (size_t)s.nsplit * sizeof(SpSplit))) {
+        sparse_free(p);
+        return fail(MI355NUFFT_ENOMEM, "sparse matrix column pieces");
+    }
+    hipLaunchKernelGGL(sp_piece_emit_kernel, dim3(nblocks(nk)), dim3(kBlock), 0, p->stream, ptr, nk, off,
+                       off + (nk + 1), off + 2 * (nk + 1), s.pieces, s.splits);
+    LAUNCH_CHECK();
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    s.on = 1;
+    return MI355NUFFT_OK;
+}
+
+// Forward with the sparse matrix.  raw: y = P xk exactly (grid_only,
+// _nufft.py:1376-1388); else phase_before and the ortho scale are applied in
+// place on the grid first (:1371-1375) and y is multiplied by phase_after
+// (:1391).
+template <typename T>
+int sparse_interp(mi355nufft_plan* p, const void* grid, void* y, int32_t nreps, bool raw) {
+    const SparseState& s = p->sp;
+    if (!raw && (p->phasing == 0 || p->ortho))
+        RET_IF(phase_grid_impl<T>(p, const_cast<void*>(grid), nreps, p->phasing == 0, 0, p->ortho));
+    if (s.M == 0) return MI355NUFFT_OK;
+    const cx<T>* ph = (!raw && p->phasing == 0 && p->has_phase) ? (const cx<T>*)p->d_phase : nullptr;
+    if (s.cvals)
+        hipLaunchKernelGGL((sp_interp_kernel<T, true>), dim3(nblocks(s.M)), dim3(kBlock), 0, p->stream,
+                           (const cx<T>*)s.val, s.col, s.perm, s.M, s.nnzr, (const cx<T>*)grid, p->nk, nreps, ph,
+                           (cx<T>*)y);
+    else
+        hipLaunchKernelGGL((sp_interp_kernel<T, false>), dim3(nblocks(s.M)), dim3(kBlock), 0, p->stream,
+                           (const T*)s.val, s.col, s.perm, s.M, s.nnzr, (const cx<T>*)grid, p->nk, nreps, ph,
+                           (cx<T>*)y);
+    LAUNCH_CHECK();
+    return MI355NUFFT_OK;
+}
+
+// Adjoint with the sparse matrix; writes every grid point.  raw: grid =
+// P^H y exactly (grid_only, :1505-1509); else y is multiplied by
+// conj(phase_after) (:1497) and the grid by conj(phase_before) (:1519-1520).
+template <typename T>
+int sparse_spread(mi355nufft_plan* p, const void* y, void* grid, int32_t nreps, bool raw) {
+    SparseState& s = p->sp;
+    const size_t need = (size_t)s.nslots * (size_t)nreps * p->cx_size();
+    if (need > s.part_bytes) {
+        if (s.part) {
+            HIP_TRY(hipStreamSynchronize(p->stream));
+            (void)hipFree(s.part);
+            s.part = nullptr;
+            s.part_bytes = 0;
+        }
+        if (hipMalloc(&s.part, need) != hipSuccess) return fail(MI355NUFFT_ENOMEM, "sparse partial sums");
+        s.part_bytes = need;
+    }
+    const cx<T>* ph = (!raw && p->phasing == 0 && p->has_phase) ? (const cx<T>*)p->d_phase : nullptr;
+    const int with_pb = (!raw && p->phasing == 0) ? 1 : 0;
+    const GridGeom g = grid_geom(p, nullptr);
+    const T c0 = (T)p->phase_coef[0], c1 = (T)p->phase_coef[1], c2 = (T)p->phase_coef[2];
+    const unsigned nb = (unsigned)((s.npieces * kSpLanes + kBlock - 1) / kBlock);
+    if (s.cvals)
+        hipLaunchKernelGGL((sp_spread_kernel<T, true>), dim3(nb), dim3(kBlock), 0, p->stream, s.pieces, s.npieces,
+                           (const cx<T>*)s.tval, s.trow, (const cx<T>*)y, s.M, nreps, ph, g, c0, c1, c2, with_pb,
+                           (cx<T>*)grid, (cx<T>*)s.part, s.nslots);
+    else
+        hipLaunchKernelGGL((sp_spread_kernel<T, false>), dim3(nb), dim3(kBlock), 0, p->stream, s.pieces, s.npieces,
+                           (const T*)s.tval, s.trow, (const cx<T>*)y, s.M, nreps, ph, g, c0, c1, c2, with_pb,
+                           (cx<T>*)grid, (cx<T>*)s.part, s.nslots);
+    LAUNCH_CHECK();
+    if (s.nsplit > 0) {
+        hipLaunchKernelGGL((sp_combine_kernel<T>), dim3(nblocks(s.nsplit)), dim3(kBlock), 0, p->stream, s.splits,
+                           s.nsplit, (const cx<T>*)s.part, s.nslots, nreps, g, c0, c1, c2, with_pb, (cx<T>*)grid);
+        LAUNCH_CHECK();
+    }
+    return MI355NUFFT_OK;
+}
+
 template <typename T>
 int run_tiled_any(mi355nufft_plan* p, bool adjoint, const void* samples_in, void* samples_out, const void* grid_in,
                   void* grid_out, int32_t nreps);
@@ -677,6 +896,7 @@ bool use_tiled(const mi355nufft_plan* p);
 
 template <typename T>
 int interp_impl(mi355nufft_plan* p, const void* grid, void* y, int32_t nreps) {
+    if (p->sp.on) return sparse_interp<T>(p, grid, y, nreps, false);
     // tiled path: phase_before and the ortho scale are applied while staging
     if (use_tiled(p)) return run_tiled_any<T>(p, false, nullptr, y, grid, nullptr, nreps);
     if (p->nframes > 1) return fail(MI355NUFFT_ESTATE, "frames need the tiled kernels");
@@ -696,6 +916,7 @@ bool use_two_phase(const mi355nufft_plan* p, int32_t nreps);
 
 template <typename T>
 int spread_impl(mi355nufft_plan* p, const void* y, void* grid, int32_t nreps, bool grid_is_zero = false) {
+    if (p->sp.on) return sparse_spread<T>(p, y, grid, nreps, false);  // writes every grid point
     if (use_tiled(p) && use_two_phase(p, nreps)) return spread_two_phase<T>(p, y, grid, nreps);
     if (!grid_is_zero) RET_IF(zero_grid<T>(grid, (int64_t)p->nframes * nreps * p->nk, p->stream));
     if (use_tiled(p)) return run_tiled_any<T>(p, true, y, nullptr, nullptr, grid, nreps);
@@ -1330,6 +1551,7 @@ int mi355nufft_plan_destroy(mi355nufft_plan* p) {
         if (p->d_sn[d]) (void)hipFree(p->d_sn[d]);
         if (p->d_tw[d]) (void)hipFree(p->d_tw[d]);
     }
+    sparse_free(p);
     if (p->d_tm) (void)hipFree(p->d_tm);
     if (p->d_phase) (void)hipFree(p->d_phase);
     if (p->d_grid) (void)hipFree(p->d_grid);
@@ -1389,6 +1611,7 @@ int mi355nufft_set_points_frames(mi355nufft_plan* p, const void* omega, int omeg
         HIP_TRY(hipMemcpy(p->d_frame_off, frame_offsets, (size_t)(nframes + 1) * sizeof(int64_t),
                           hipMemcpyHostToDevice));
     }
+    sparse_free(p);  // the sparse matrix belongs to the previous points
     p->nframes = nframes;
     if (M < 0 || M >= ((int64_t)1 << 31)) return fail(MI355NUFFT_EINVAL, "M must be in [0, 2^31)");
     if (M > 0 && !omega) return fail(MI355NUFFT_EINVAL, "omega is null");
@@ -1490,11 +1713,11 @@ int mi355nufft_plan_info(const mi355nufft_plan* p, int64_t* info, int32_t n) {
     g_err.clear();
     RET_IF(check_plan(p));
     if (!info || n < 1) return fail(MI355NUFFT_EINVAL, "info buffer required");
-    const int64_t v[16] = {p->ndim, p->M, p->bg.B[0], p->bg.B[1], p->bg.B[2], p->bg.P, p->bg.nbins, p->bg.tile,
+    const int64_t v[18] = {p->ndim, p->M, p->bg.B[0], p->bg.B[1], p->bg.B[2], p->bg.P, p->bg.nbins, p->bg.tile,
                            use_tiled(p) ? 1 : 0, p->binned ? max_subproblems(p) : 0, p->method, (int64_t)p->lds_limit,
                            p->nframes, (use_tiled(p) && use_two_phase(p, 1)) ? 1 : 0, p->bg_threads,
-                           use_fused_fft(p) ? 1 : 0};
-    for (int i = 0; i < n && i < 16; ++i) info[i] = v[i];
+                           use_fused_fft(p) ? 1 : 0, p->sp.on ? p->sp.nnz : -1, p->sp.on ? p->sp.npieces : 0};
+    for (int i = 0; i < n && i < 18; ++i) info[i] = v[i];
     return MI355NUFFT_OK;
 }
 
@@ -1541,6 +1764,7 @@ int mi355nufft_interp_grad(mi355nufft_plan* p, const void* grid, const void* w, 
     RET_IF(check_plan(p));
     RET_IF(check_nreps(nreps));
     RET_IF(need_points(p));
+    if (p->sp.on) return fail(MI355NUFFT_ESTATE, "the trajectory gradient needs table mode (no sparse matrix)");
     if (p->M == 0) return MI355NUFFT_OK;
     if (!grid || !w || !grad) return fail(MI355NUFFT_EINVAL, "grid, w and grad are required");
     if (!use_tiled(p))
@@ -1575,6 +1799,42 @@ int mi355nufft_adjoint_image(mi355nufft_plan* p, void* grid, void* x, const int6
     if (!x || !xs || !grid) return fail(MI355NUFFT_EINVAL, "grid, x and xs are required");
     return p->precision ? adjoint_image_impl<double>(p, grid, x, xs, nimg, 0)
                         : adjoint_image_impl<float>(p, grid, x, xs, nimg, 0);
+}
+
+int mi355nufft_set_sparse(mi355nufft_plan* p, const void* const* coef, const int32_t* const* kidx,
+                          const void* phase) {
+    g_err.clear();
+    RET_IF(check_plan(p));
+    RET_IF(need_points(p));
+    sparse_free(p);
+    if (p->M > 0) {
+        if (!coef || !kidx) return fail(MI355NUFFT_EINVAL, "coef and kidx are required");
+        for (int d = 0; d < p->ndim; ++d)
+            if (!coef[d] || !kidx[d]) return fail(MI355NUFFT_EINVAL, "coef[%d] and kidx[%d] are required", d, d);
+    }
+    if (phase && p->phasing == 0) return fail(MI355NUFFT_EINVAL, "phase is for the complex kernel (phasing 1)");
+    if (p->precision)
+        return p->phasing ? sparse_build<double, true>(p, coef, kidx, phase)
+                          : sparse_build<double, false>(p, coef, kidx, phase);
+    return p->phasing ? sparse_build<float, true>(p, coef, kidx, phase) : sparse_build<float, false>(p, coef, kidx, phase);
+}
+
+int mi355nufft_clear_sparse(mi355nufft_plan* p) {
+    g_err.clear();
+    RET_IF(check_plan(p));
+    sparse_free(p);
+    return MI355NUFFT_OK;
+}
+
+int mi355nufft_sparse_apply(mi355nufft_plan* p, int adjoint, const void* in, void* out, int32_t nreps) {
+    g_err.clear();
+    RET_IF(check_plan(p));
+    RET_IF(check_nreps(nreps));
+    if (!p->sp.on) return fail(MI355NUFFT_ESTATE, "no sparse matrix (mi355nufft_set_sparse)");
+    if (!out || (!in && (adjoint ? p->M > 0 : true))) return fail(MI355NUFFT_EINVAL, "in and out are required");
+    if (adjoint)
+        return p->precision ? sparse_spread<double>(p, in, out, nreps, true) : sparse_spread<float>(p, in, out, nreps, true);
+    return p->precision ? sparse_interp<double>(p, in, out, nreps, true) : sparse_interp<float>(p, in, out, nreps, true);
 }
 
 int mi355nufft_set_fft(mi355nufft_plan* p, int method) {

@@ -158,6 +158,57 @@ def _sparse_1d_matrix(omega, N, J, K, phasing):
     return p.tocsc()
 
 
+def sparse_matrix(omega_rdt, Nd, Jd, Kd, alpha, phasing, n_shift, rdt):
+    """Interpolation matrix of ``mode='sparse'``: ``_init_sparsemat``
+    (``_nufft.py:751-877``) for any ndim, as a scipy CSC matrix (M, prod Kd).
+
+    ``omega_rdt`` is omega already cast to the real dtype (``:315`` runs
+    before ``:322``).  Products in float64 / complex128, conjugated for the
+    complex kernel, times exp(i omega . n_shift), then cast to the matrix
+    dtype (real dtype, or complex dtype for the complex kernel).
+    """
+    omega = np.asarray(omega_rdt)
+    if omega.ndim == 1:
+        omega = omega[:, None]
+    ndim = omega.shape[1]
+    M = omega.shape[0]
+    ud, kd = [], []
+    for d in range(ndim):                                            # :775-802
+        N, J, K = Nd[d], Jd[d], Kd[d]
+        c, arg = nufft_coef(omega[:, d], J, K, alpha[d])
+        koff = nufft_offset(omega[:, d], J, K)
+        kd.append(np.mod(np.arange(1, J + 1)[:, None] + koff[None, :], K))
+        if phasing == "complex":
+            gam = 2 * np.pi / K
+            phase_scale = 1j * gam * (N - 1) / 2.0
+            phase = np.exp(phase_scale * arg)
+        else:
+            phase = 1.0
+        ud.append(phase * c)
+    kk, uu = kd[0], ud[0]                                            # :812-823
+    for d in range(1, ndim):
+        Jprod = int(np.prod(Jd[: d + 1]))
+        tmp = kd[d] * int(np.prod(Kd[:d]))
+        kk = (kk[:, None, :] + tmp[None, :, :]).reshape((Jprod, M), order="F")
+        uu = (uu[:, None, :] * ud[d][None, :, :]).reshape((Jprod, M), order="F")
+    if np.iscomplexobj(uu):                                          # :830-831
+        uu = uu.conj()
+    if phasing == "complex":                                         # :833-840
+        if any(s != 0 for s in n_shift):
+            phase = np.exp(1j * np.dot(omega, np.asarray(n_shift))).reshape((1, -1), order="F")
+            uu = uu * phase
+        sparse_dtype = np.complex64 if np.dtype(rdt) == np.float32 else np.complex128
+    else:
+        sparse_dtype = rdt
+    mm = np.tile(np.arange(M), (int(np.prod(Jd)), 1))               # :852
+    p = scipy.sparse.coo_matrix(
+        (uu.ravel(order="F"), (mm.ravel(order="F"), kk.astype(np.int64).ravel(order="F"))),
+        shape=(M, int(np.prod(Kd))),
+        dtype=sparse_dtype,
+    )
+    return p.tocsc()
+
+
 def table_make1(how, N, J, K, L, phasing="real"):
     """Interpolation table ``h[J*L+1]``, ``_nufft.py:1195-1271``.
 
@@ -441,7 +492,7 @@ class NufftOracle:
 
     def __init__(self, Nd, omega, Jd=4, Kd=None, precision="single", Ld=1024,
                  ortho=False, n_shift=None, phasing="real",
-                 adjoint_scalefactor=1.0, interp=None):
+                 adjoint_scalefactor=1.0, interp=None, mode="table"):
         Nd = (Nd,) if np.isscalar(Nd) else tuple(int(n) for n in Nd)
         ndim = len(Nd)
         Jd = (int(Jd),) * ndim if np.isscalar(Jd) else tuple(int(j) for j in Jd)
@@ -480,6 +531,19 @@ class NufftOracle:
             self.phase_after = None
         self.omega = omega.astype(self.rdt)
         self.M = self.omega.shape[0]
+        self.mode = mode
+        if mode == "sparse":
+            # _init_sparsemat re-runs _set_phase_funcs on the cast omega
+            # (:768-769); the complex kernel's shift phase goes into P (:833-838)
+            if phasing == "real":
+                self.phase_after = phase_after(self.omega, self.n_mid, self.n_shift, self.rdt)
+            self.phase_shift = None
+            self.p = sparse_matrix(self.omega, Nd, Jd, Kd, self.alpha, phasing, self.n_shift, self.rdt)
+            self._fwd = lambda ck: np.asarray(self.p @ ck)          # :1384
+            self._adj = lambda fm: np.asarray(self.p.getH() @ fm)   # :1505
+            self.h = None
+            self.tm = compute_tm(self.omega, Kd)
+            return
         if phasing == "complex" and any(s != 0 for s in self.n_shift):
             # _nufft.py:898-901: from the cast omega, kept complex128
             self.phase_shift = np.exp(1j * np.dot(self.omega, np.asarray(self.n_shift)))

@@ -201,16 +201,18 @@ def test_oracle_nufft_adj_kat(phasing):
     np.testing.assert_array_almost_equal(xd, xn[:, :, -1], decimal=4)
 
 
+@pytest.mark.parametrize("mode", ["table", "sparse"])
 @pytest.mark.parametrize("precision", ["single", "double"])
 @pytest.mark.parametrize("phasing", ["real", "complex"])
 @pytest.mark.parametrize("Kd,Jd", [((32, 32), 6), ((33, 31), 7)])
-def test_oracle_nufft_2d_vs_dtft(precision, phasing, Kd, Jd):
-    """mrrt/nufft/tests/test_nufft.py:172-244 (table mode, order F)."""
+def test_oracle_nufft_2d_vs_dtft(precision, phasing, Kd, Jd, mode):
+    """mrrt/nufft/tests/test_nufft.py:172-244 (order F)."""
     Nd = (16, 16)
     n_shift = np.asarray(Nd) / 2
     om = perturbed_gridpoints(Nd)
     rs = np.random.RandomState(1234)
-    A = O.NufftOracle(Nd, om, Jd=Jd, Kd=Kd, n_shift=n_shift, Ld=1024, precision=precision, phasing=phasing)
+    A = O.NufftOracle(Nd, om, Jd=Jd, Kd=Kd, n_shift=n_shift, Ld=1024, precision=precision, phasing=phasing,
+                      mode=mode)
     x = rs.standard_normal(Nd) + 1j * rs.standard_normal(Nd)
     y = A.fft(x)
     np.testing.assert_allclose(y, O.dtft(x.ravel(order="F"), om, Nd, n_shift), rtol=1e-3, atol=1e-5)
@@ -218,12 +220,13 @@ def test_oracle_nufft_2d_vs_dtft(precision, phasing, Kd, Jd):
     np.testing.assert_allclose(xa, O.dtft_adj(y, om, Nd, n_shift).reshape(Nd, order="F"), rtol=1e-3, atol=1e-5)
 
 
+@pytest.mark.parametrize("mode", ["table", "sparse"])
 @pytest.mark.parametrize("phasing", ["real", "complex"])
-def test_oracle_nufft_1d_3d_vs_dtft(phasing):
-    """mrrt/nufft/tests/test_nufft.py:99-169 and :247-324 (table mode)."""
+def test_oracle_nufft_1d_3d_vs_dtft(phasing, mode):
+    """mrrt/nufft/tests/test_nufft.py:99-169 and :247-324."""
     om = perturbed_gridpoints(64)
     rs = np.random.RandomState(1234)
-    A = O.NufftOracle(64, om, Jd=6, Kd=128, n_shift=32, Ld=1024, precision="double", phasing=phasing)
+    A = O.NufftOracle(64, om, Jd=6, Kd=128, n_shift=32, Ld=1024, precision="double", phasing=phasing, mode=mode)
     x = rs.randn(64) + 1j * rs.randn(64)
     y = A.fft(x)
     np.testing.assert_allclose(y, O.dtft(x, om, (64,), [32]), rtol=1e-3, atol=1e-5)
@@ -231,7 +234,7 @@ def test_oracle_nufft_1d_3d_vs_dtft(phasing):
     Nd = (8, 8, 8)
     om = perturbed_gridpoints(Nd)
     A = O.NufftOracle(Nd, om, Jd=6, Kd=(16, 16, 16), n_shift=np.asarray(Nd) / 2, Ld=1024,
-                      precision="single", phasing=phasing)
+                      precision="single", phasing=phasing, mode=mode)
     x = rs.standard_normal(Nd) + 1j * rs.standard_normal(Nd)
     y = A.fft(x)
     np.testing.assert_allclose(y, O.dtft(x.ravel(order="F"), om, Nd, np.asarray(Nd) / 2), rtol=1e-2, atol=1e-4)
@@ -264,3 +267,57 @@ def test_phase_before_single_quirk():
     assert np.max(np.abs(pb64 - exact)) < 1e-10
     err = np.angle(pb32 / exact)
     assert 5e-6 < np.sqrt(np.mean(err ** 2)) < 1e-4   # the quirk is present, and bounded
+
+
+# ------------------------------------------------------------- sparse mode
+@pytest.mark.parametrize("phasing", ["real", "complex"])
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+def test_sparse_matrix_bruteforce_tiny(phasing, ndim):
+    """Every entry of the sparse matrix (_nufft.py:751-877) against a sum
+    over the whole grid, without offsets or tap order:
+    P[m, k] = prod_d sum_n g_d(t_d - k_d - n K_d), t_d = omega_d / (2 pi / K_d),
+    g = KB on |x| < J/2 (times exp(-i pi (N-1) x / K), the conjugated phase of
+    :793-796 / :830-831, for the complex kernel), times exp(i omega . n_shift)
+    (complex kernel, :833-838).  k is the Fortran flat index."""
+    Nd, Kd, Jd, n_shift = [(5, 4, 3), (8, 7, 6), (4, 3, 5), (1.5, -0.5, 2.0)]
+    Nd, Kd, Jd, n_shift = Nd[:ndim], Kd[:ndim], Jd[:ndim], n_shift[:ndim]
+    M = 6
+    om = np.random.default_rng(5).uniform(-np.pi, np.pi, (M, ndim))
+    alpha = [O.beatty_alpha(Jd[d], Kd[d], Nd[d]) for d in range(ndim)]
+    P = O.sparse_matrix(om, Nd, Jd, Kd, alpha, phasing, n_shift, np.float64).toarray()
+    assert P.shape == (M, int(np.prod(Kd)))
+
+    def g(d, x):
+        if not 2 * abs(x) < Jd[d]:
+            return 0.0
+        v = i0(alpha[d] * np.sqrt(1 - (2 * x / Jd[d]) ** 2)) / i0(alpha[d])
+        if phasing == "complex":
+            v = v * np.exp(-1j * np.pi * (Nd[d] - 1) * x / Kd[d])
+        return v
+
+    for m in range(M):
+        for k in range(int(np.prod(Kd))):
+            kk = np.unravel_index(k, Kd, order="F")
+            v = 1.0 + 0j
+            for d in range(ndim):
+                t = om[m, d] / (2 * np.pi / Kd[d])
+                v *= sum(g(d, t - kk[d] - n * Kd[d]) for n in range(-2, 3))
+            if phasing == "complex":
+                v *= np.exp(1j * np.dot(om[m], np.asarray(n_shift)))
+            assert abs(P[m, k] - v) <= 1e-12 * max(1.0, abs(v)), (m, k, P[m, k], v)
+
+
+def test_sparse_matrix_single_precision_quirks():
+    """float32 operator: the matrix is built from the cast omega and stored
+    as float32 (:315, :843, :854-858); phase_after then comes from the cast
+    omega too (:768-769), unlike table mode (:313)."""
+    Nd, Kd = (16, 16), (32, 32)
+    om = perturbed_gridpoints(Nd)
+    A = O.NufftOracle(Nd, om, Jd=6, Kd=Kd, precision="single", mode="sparse")
+    B = O.NufftOracle(Nd, om, Jd=6, Kd=Kd, precision="single", mode="table")
+    assert A.p.dtype == np.float32
+    alpha = [O.beatty_alpha(6, 32, 16)] * 2
+    P64 = O.sparse_matrix(om.astype(np.float32), Nd, (6, 6), Kd, alpha, "real", (0, 0), np.float64)
+    np.testing.assert_array_equal(A.p.toarray(), P64.toarray().astype(np.float32))
+    np.testing.assert_array_equal(A.phase_after, O.phase_after(om.astype(np.float32), A.n_mid, A.n_shift, np.float32))
+    assert not np.array_equal(A.phase_after, B.phase_after)
