@@ -265,8 +265,25 @@ def run_c5(steps=5):
 
     t_ad = _time_fn(ad_step, steps, 2)
     t_pl = _time_fn(plain, steps, 2)
-    return {"c5_autograd_fwd_bwd_ms": round(t_ad, 4), "c5_plain_fwd_adj_ms": round(t_pl, 4),
-            "c5_autograd_overhead": round(t_ad / t_pl, 3)}
+    out = {"c5_autograd_fwd_bwd_ms": round(t_ad, 4), "c5_plain_fwd_adj_ms": round(t_pl, 4),
+           "c5_autograd_overhead": round(t_ad / t_pl, 3)}
+    # SENSE encoding of one image with 8 coil maps and density weights:
+    # E x, E^H W y, and the normal operator E^H W E x directly and through
+    # the Toeplitz operator
+    from mrrt.nufft_amd.mri import SenseOperator
+    g = np.meshgrid(np.linspace(-1, 1, Nd[0]), np.linspace(-1, 1, Nd[1]), indexing="ij")
+    cen = np.random.default_rng(2).uniform(-1, 1, (C, 2))
+    smaps = np.stack([np.exp(-((g[0] - a) ** 2 + (g[1] - b) ** 2)) for a, b in cen]).astype(np.complex64)
+    w = np.random.default_rng(3).uniform(0.5, 1.0, om.shape[0]).astype(np.float32)
+    E = SenseOperator(op, smaps, weights=w)
+    xi = x.detach()[0].contiguous()
+    ys = E.forward_dev(xi)
+    t_fa = _time_fn(lambda: E.adjoint_dev(E.forward_dev(xi)), steps, 2)
+    E.normal_dev(xi, toeplitz=True)
+    t_tn = _time_fn(lambda: E.normal_dev(xi, toeplitz=True), steps, 2)
+    del ys
+    out.update({"c5_sense_8coil_fwd_adj_ms": round(t_fa, 4), "c5_sense_8coil_normal_toeplitz_ms": round(t_tn, 4)})
+    return out
 
 
 def run_sparse(wl, tag, steps, warmup, fft):

@@ -312,6 +312,30 @@ int mi355nufft_clear_sparse(mi355nufft_plan *plan);
 int mi355nufft_sparse_apply(mi355nufft_plan *plan, int adjoint, const void *in,
                             void *out, int32_t nreps);
 
+/* ---- multi-coil encoding (no plan) ---- */
+
+/*
+ * The multi-coil (SENSE) steps a caller wraps around the transforms.  The
+ * reference leaves them to MRI_Operator of the external mrrt.operators
+ * (recommended in mrrt/nufft/_nufft.py:3-5; not part of the tree).  Images
+ * are flat, pixel n of image i at [i*npix + n], with one pixel order shared
+ * by images and maps.  precision 0: complex64 / float32, 1: complex128 /
+ * float64.  Enqueued on stream.
+ *
+ * coil_expand:  out[(f*C + c)*npix + n] = smaps[c*npix + n] * x[f*npix + n]
+ *               for nimg images f and C = ncoils maps
+ * coil_combine: x[f*npix + n] = sum_c conj(smaps[c*npix + n]) * img[(f*C + c)*npix + n]
+ * weight_samples: out[r*M + m] = w[m] * y[r*M + m] (real density weights)
+ */
+int mi355nufft_coil_expand(int precision, const void *x, const void *smaps,
+                           int64_t npix, int32_t nimg, int32_t ncoils,
+                           void *out, void *stream);
+int mi355nufft_coil_combine(int precision, const void *img, const void *smaps,
+                            int64_t npix, int32_t nimg, int32_t ncoils,
+                            void *x, void *stream);
+int mi355nufft_weight_samples(int precision, const void *y, const void *w,
+                              int64_t M, int32_t nreps, void *out, void *stream);
+
 /* ---- raw table1 kernels (no plan) ---- */
 
 /*

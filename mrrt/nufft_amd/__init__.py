@@ -1,9 +1,11 @@
-"""MI355X-native table1 NUFFT (drop-in for mrrt.nufft's table mode).
+"""MI355X-native table1 NUFFT (drop-in for mrrt.nufft's table and sparse modes).
 
-Public names mirror ``mrrt.nufft`` (mrrt/nufft/__init__.py:1-15) for the
-table-mode path: ``NufftBase``, ``nufft_forward``, ``nufft_adj``,
-``kaiser_bessel``, ``kaiser_bessel_ft``, ``BeattyKernel`` parameters via
-``beatty_alpha``.
+Public names mirror ``mrrt.nufft`` (mrrt/nufft/__init__.py:1-15):
+``NufftBase``, ``nufft_forward``, ``nufft_adj``, ``kaiser_bessel``,
+``kaiser_bessel_ft``, ``BeattyKernel`` parameters via ``beatty_alpha``.
+Beyond them: ``batched.NufftFrames`` (frames x coils), ``autograd``,
+``toeplitz.ToeplitzNormal``, ``mri.SenseOperator`` (multi-coil encoding)
+and ``autotune``.
 """
 
 from ._kernel_math import beatty_alpha, kaiser_bessel, kaiser_bessel_ft  # noqa: F401

@@ -62,6 +62,9 @@ _SIGNATURES = {
     "mi355nufft_set_fft": (c_int, [c_vp, c_int]),
     "mi355nufft_set_sparse": (c_int, [c_vp, c_vp, c_vp, c_vp]),
     "mi355nufft_clear_sparse": (c_int, [c_vp]),
+    "mi355nufft_coil_expand": (c_int, [c_int, c_vp, c_vp, c_i64, c_i32, c_i32, c_vp, c_vp]),
+    "mi355nufft_coil_combine": (c_int, [c_int, c_vp, c_vp, c_i64, c_i32, c_i32, c_vp, c_vp]),
+    "mi355nufft_weight_samples": (c_int, [c_int, c_vp, c_vp, c_i64, c_i32, c_vp, c_vp]),
     "mi355nufft_sparse_apply": (c_int, [c_vp, c_int, c_vp, c_vp, c_i32]),
     "mi355nufft_toeplitz_create": (c_int, [ctypes.POINTER(c_vp), c_int, c_i64p, c_int, c_vp]),
     "mi355nufft_toeplitz_destroy": (c_int, [c_vp]),

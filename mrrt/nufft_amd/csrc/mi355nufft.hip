@@ -21,6 +21,7 @@
 #include "kernels_binned.hpp"
 #include "kernels_fft.hpp"
 #include "kernels_sparse.hpp"
+#include "kernels_coil.hpp"
 
 using namespace mi355;
 
@@ -1835,6 +1836,65 @@ int mi355nufft_sparse_apply(mi355nufft_plan* p, int adjoint, const void* in, voi
     if (adjoint)
         return p->precision ? sparse_spread<double>(p, in, out, nreps, true) : sparse_spread<float>(p, in, out, nreps, true);
     return p->precision ? sparse_interp<double>(p, in, out, nreps, true) : sparse_interp<float>(p, in, out, nreps, true);
+}
+
+int mi355nufft_coil_expand(int precision, const void* x, const void* smaps, int64_t npix, int32_t nimg,
+                           int32_t ncoils, void* out, void* stream) {
+    g_err.clear();
+    RET_IF(check_nreps(nimg));
+    if (precision != 0 && precision != 1) return fail(MI355NUFFT_EINVAL, "precision must be 0 or 1");
+    if (ncoils < 1 || (int64_t)nimg * ncoils > 65535)
+        return fail(MI355NUFFT_EINVAL, "ncoils must be >= 1 with nimg * ncoils <= 65535");
+    if (npix < 1 || npix >= ((int64_t)1 << 40)) return fail(MI355NUFFT_EINVAL, "npix out of range");
+    if (!x || !smaps || !out) return fail(MI355NUFFT_EINVAL, "x, smaps and out are required");
+    const dim3 grid(nblocks(npix), nimg);
+    if (precision)
+        hipLaunchKernelGGL(coil_expand_kernel<double>, grid, dim3(kBlock), 0, (hipStream_t)stream,
+                           (const cx<double>*)x, (const cx<double>*)smaps, npix, ncoils, (cx<double>*)out);
+    else
+        hipLaunchKernelGGL(coil_expand_kernel<float>, grid, dim3(kBlock), 0, (hipStream_t)stream,
+                           (const cx<float>*)x, (const cx<float>*)smaps, npix, ncoils, (cx<float>*)out);
+    LAUNCH_CHECK();
+    return MI355NUFFT_OK;
+}
+
+int mi355nufft_coil_combine(int precision, const void* img, const void* smaps, int64_t npix, int32_t nimg,
+                            int32_t ncoils, void* x, void* stream) {
+    g_err.clear();
+    RET_IF(check_nreps(nimg));
+    if (precision != 0 && precision != 1) return fail(MI355NUFFT_EINVAL, "precision must be 0 or 1");
+    if (ncoils < 1 || (int64_t)nimg * ncoils > 65535)
+        return fail(MI355NUFFT_EINVAL, "ncoils must be >= 1 with nimg * ncoils <= 65535");
+    if (npix < 1 || npix >= ((int64_t)1 << 40)) return fail(MI355NUFFT_EINVAL, "npix out of range");
+    if (!img || !smaps || !x) return fail(MI355NUFFT_EINVAL, "img, smaps and x are required");
+    const dim3 grid(nblocks(npix), nimg);
+    if (precision)
+        hipLaunchKernelGGL(coil_combine_kernel<double>, grid, dim3(kBlock), 0, (hipStream_t)stream,
+                           (const cx<double>*)img, (const cx<double>*)smaps, npix, ncoils, (cx<double>*)x);
+    else
+        hipLaunchKernelGGL(coil_combine_kernel<float>, grid, dim3(kBlock), 0, (hipStream_t)stream,
+                           (const cx<float>*)img, (const cx<float>*)smaps, npix, ncoils, (cx<float>*)x);
+    LAUNCH_CHECK();
+    return MI355NUFFT_OK;
+}
+
+int mi355nufft_weight_samples(int precision, const void* y, const void* w, int64_t M, int32_t nreps, void* out,
+                              void* stream) {
+    g_err.clear();
+    RET_IF(check_nreps(nreps));
+    if (precision != 0 && precision != 1) return fail(MI355NUFFT_EINVAL, "precision must be 0 or 1");
+    if (M < 0 || M >= ((int64_t)1 << 40)) return fail(MI355NUFFT_EINVAL, "M out of range");
+    if (M == 0) return MI355NUFFT_OK;
+    if (!y || !w || !out) return fail(MI355NUFFT_EINVAL, "y, w and out are required");
+    const dim3 grid(nblocks(M), nreps);
+    if (precision)
+        hipLaunchKernelGGL(weight_samples_kernel<double>, grid, dim3(kBlock), 0, (hipStream_t)stream,
+                           (const cx<double>*)y, (const double*)w, M, (cx<double>*)out);
+    else
+        hipLaunchKernelGGL(weight_samples_kernel<float>, grid, dim3(kBlock), 0, (hipStream_t)stream,
+                           (const cx<float>*)y, (const float*)w, M, (cx<float>*)out);
+    LAUNCH_CHECK();
+    return MI355NUFFT_OK;
 }
 
 int mi355nufft_set_fft(mi355nufft_plan* p, int method) {

@@ -103,13 +103,17 @@ class ToeplitzNormal:
             _lib.check(self._lib.mi355nufft_toeplitz_set_stream(self._plan, ctypes.c_void_p(s)))
             self._stream = s
 
-    def apply_dev(self, x):
+    def apply_dev(self, x, out=None):
         """x: device tensor with logical shape Nd + (nreps,) (any strides);
-        returns A^H W A x with the same logical shape, Fortran-ordered."""
+        returns A^H W A x with the same logical shape, Fortran-ordered, or
+        written into ``out`` (same logical shape, any strides)."""
         nreps = x.shape[-1]
         x = x.to(self._tcdt)
-        out = torch.empty((nreps,) + tuple(reversed(self.Nd)), dtype=self._tcdt, device=self.device)
-        out = out.permute(*(tuple(range(self.ndim, 0, -1)) + (0,)))
+        if out is None:
+            out = torch.empty((nreps,) + tuple(reversed(self.Nd)), dtype=self._tcdt, device=self.device)
+            out = out.permute(*(tuple(range(self.ndim, 0, -1)) + (0,)))
+        elif tuple(out.shape) != tuple(x.shape) or out.dtype != self._tcdt:
+            raise ValueError("out must have the logical shape and dtype of the result")
         self._sync_stream()
         _lib.check(self._lib.mi355nufft_toeplitz_apply(
             self._plan, ctypes.c_void_p(x.data_ptr()), _lib.i64arr(_fortran_strides(x, self.ndim)),

@@ -119,3 +119,24 @@ def test_operator_argument_errors():
         NufftBase((8, 8), om, Jd=6, Kd=(16, 16), precision="half", device=dev)
     with pytest.raises(ValueError):
         NufftBase((8, 8), om, Jd=(6, 6, 6), Kd=(16, 16), device=dev)
+
+
+@pytest.mark.parametrize("call,msg", [
+    (lambda L: L.mi355nufft_coil_expand(0, None, None, 64, 1, 4, None, None), "required"),
+    (lambda L: L.mi355nufft_coil_expand(2, None, None, 64, 1, 4, None, None), "precision"),
+    (lambda L: L.mi355nufft_coil_expand(0, None, None, 64, 1, 0, None, None), "ncoils"),
+    (lambda L: L.mi355nufft_coil_combine(0, None, None, 64, 20000, 4, None, None), "ncoils"),
+    (lambda L: L.mi355nufft_coil_combine(1, None, None, 0, 1, 4, None, None), "npix"),
+    (lambda L: L.mi355nufft_weight_samples(0, None, None, 10, 0, None, None), "nreps"),
+    (lambda L: L.mi355nufft_weight_samples(0, None, None, 10, 1, None, None), "required"),
+])
+def test_coil_entry_points_reject_without_device_work(call, msg):
+    lib = _lib.get_lib()
+    assert call(lib) == 1 and msg in lib.mi355nufft_last_error().decode()
+
+
+def test_sparse_entry_points_reject_null_plan():
+    lib = _lib.get_lib()
+    assert lib.mi355nufft_set_sparse(None, None, None, None) == 1
+    assert lib.mi355nufft_clear_sparse(None) == 1
+    assert lib.mi355nufft_sparse_apply(None, 0, None, None, 1) == 1
