@@ -458,7 +458,7 @@ def main():
         extra.update(run_c4())
         torch.cuda.empty_cache()
         print("bench: autograd spiral", file=sys.stderr, flush=True)
-        extra.update(run_c5())
+        extra.update(run_c5(steps=20))
     cpu = None
     if not args.no_cpu_baseline:
         print("bench: cpu baseline (~%.0f s)" % args.cpu_seconds, file=sys.stderr, flush=True)
