@@ -30,6 +30,7 @@ struct BinGeom {
     int nbins;      // bins of all frames: nframes * nbins_frame
     int nbins_frame;  // nb0 * nb1 * nb2
     int P;          // max points per subproblem
+    float invT[3];  // 1 / T[d], for the tile index split (tile_to_grid)
 };
 
 struct BinState {
@@ -246,16 +247,30 @@ __device__ __forceinline__ void load_sub(const BinGeom& bg, const BinState& bs, 
     }
 }
 
+// floor(e / t) for 0 <= e < 2^22 through the float reciprocal: (e + 0.5) / t
+// lies at least 0.5 / t from an integer, farther than the 2^-23 relative
+// rounding of the product, so the truncation is exact (tiles are far below
+// 2^22 points).  Replaces an integer division, which the GPU does in
+// software.
+__device__ __forceinline__ int div_small(int e, float inv_t) { return (int)(((float)e + 0.5f) * inv_t); }
+
+// (o + e) mod K for 0 <= o < K, e >= 0
+__device__ __forceinline__ int wrap_add(int o, int e, int K) {
+    int g = o + e;
+    while (g >= K) g -= K;
+    return g;
+}
+
 // grid index of local tile point e (local coordinates e0, e1, e2)
 __device__ __forceinline__ int64_t tile_to_grid(int e, const BinGeom& bg, const Sub& s, const AxisGeom* ax,
                                                 int& g0, int& g1, int& g2, int& e0, int& e1, int& e2) {
-    e0 = e % bg.T[0];
-    const int q = e / bg.T[0];
-    e1 = q % bg.T[1];
-    e2 = q / bg.T[1];
-    g0 = (s.o[0] + e0) % ax[0].K;
-    g1 = (s.o[1] + e1) % ax[1].K;
-    g2 = (s.o[2] + e2) % ax[2].K;
+    const int q = div_small(e, bg.invT[0]);
+    e0 = e - q * bg.T[0];
+    e2 = div_small(q, bg.invT[1]);
+    e1 = q - e2 * bg.T[1];
+    g0 = wrap_add(s.o[0], e0, ax[0].K);
+    g1 = wrap_add(s.o[1], e1, ax[1].K);
+    g2 = wrap_add(s.o[2], e2, ax[2].K);
     return ((int64_t)g2 * ax[1].K + g1) * ax[0].K + g0;
 }
 

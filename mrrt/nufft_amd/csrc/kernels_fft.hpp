@@ -285,7 +285,45 @@ struct FftColGeom {
     int lpw;          // neighbouring lines per workgroup, divides S (blockDim = lpw * TL)
     int zero_tail;    // also write zeros at n_out <= t < K
     const void* mult; // complex T grid (one image) multiplying the loaded lines, or null
+    // Last forward pass / first adjoint pass of an ndim >= 2 grid: pb = 1
+    // multiplies what the pass writes by phase_before (then divides by
+    // scale_ortho if ortho), pb = 2 multiplies what it loads by
+    // conj(phase_before); the element's axis-0 (and axis-1) index comes from
+    // the inner line index i0 = k0 + K0 * k1 (K0 = k0_mask + 1, a power of two).
+    int pb;
+    int ortho;
+    int pb_ndim;
+    int k0_mask, k0_shift;
+    double pbc[3];      // phase_before coefficients, rounded to T in the kernel
+    double scale_ortho;
 };
+
+// grid value times phase_before (then / scale_ortho): the arithmetic of the
+// tiled gather's tile staging, so the staged values are unchanged
+template <typename T>
+__device__ __forceinline__ cx<T> fft_pb_apply(cx<T> w, const FftColGeom& g, int64_t i0, int t, bool conj) {
+    const T c[3] = {(T)g.pbc[0], (T)g.pbc[1], (T)g.pbc[2]};
+    int k0, k1, k2;
+    if (g.pb_ndim == 2) {
+        k0 = (int)i0;
+        k1 = t;
+        k2 = 0;
+    } else {
+        k0 = (int)(i0 & g.k0_mask);
+        k1 = (int)(i0 >> g.k0_shift);
+        k2 = t;
+    }
+    cx<T> p = cis_tile(phase_before_arg<T>(c, g.pb_ndim, k0, k1, k2));
+    if (conj) p.y = -p.y;
+    cx<T> o;
+    o.x = w.x * p.x - w.y * p.y;
+    o.y = w.x * p.y + w.y * p.x;
+    if (!conj && g.ortho) {
+        o.x = o.x / (T)g.scale_ortho;
+        o.y = o.y / (T)g.scale_ortho;
+    }
+    return o;
+}
 
 // Axis d >= 1 in place: lpw neighbouring lines (inner index i0..i0+lpw-1) of
 // one outer index per workgroup; each row of the block is one contiguous
@@ -306,18 +344,24 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_cols_kernel(cx<T>* __restr
     const int64_t base = ibase + (o / g.A) * g.nk;
     const int li = threadIdx.x % lpw, t0 = threadIdx.x / lpw;  // load / store mapping
     const cx<T>* mul = (const cx<T>*)g.mult;
+    const int64_t i0 = chunk * lpw + li;  // inner index of this thread's line
     for (int t = t0; t < K; t += TL) {
         cx<T> v = mkcx<T>((T)0, (T)0);
         if (t < g.n_in) {
             v = grid[base + (int64_t)t * g.S + li];
             if (mul) v = cmulc<T>(v, mul[ibase + (int64_t)t * g.S + li]);
+            if (g.pb == 2) v = fft_pb_apply<T>(v, g, i0, t, true);
         }
         lds[li * C::KP + pad(t)] = v;
     }
     cx<T>* tws = stage_twiddles<T, K>(smem, g.lpw, tw);
     __syncthreads();
     fft_line<T, K, TL, SIGN>(lds + (threadIdx.x / TL) * C::KP, threadIdx.x % TL, tws);
-    for (int t = t0; t < g.n_out; t += TL) grid[base + (int64_t)t * g.S + li] = lds[li * C::KP + pad(t)];
+    for (int t = t0; t < g.n_out; t += TL) {
+        cx<T> v = lds[li * C::KP + pad(t)];
+        if (g.pb == 1) v = fft_pb_apply<T>(v, g, i0, t, false);
+        grid[base + (int64_t)t * g.S + li] = v;
+    }
     if (g.zero_tail)
         for (int t = g.n_out + t0; t < K; t += TL) grid[base + (int64_t)t * g.S + li] = mkcx<T>((T)0, (T)0);
 }

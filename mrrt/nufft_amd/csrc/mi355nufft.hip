@@ -105,6 +105,10 @@ struct mi355nufft_plan {
     void* d_tw[3] = {nullptr, nullptr, nullptr};   // FFT twiddles exp(-2 pi i m / K_d), complex T
     int fft_fused_ok = 0;  // every K_d a power of two in [kFftMinK, kFftMaxK]
     int fft_method = 0;    // 0 auto (fused pruned FFT when possible), 1 rocFFT
+    // set while mi355nufft_forward / _adjoint run with phase_before (and the
+    // forward's ortho scale) applied by the fused FFT passes instead of the
+    // tiled gather's staging / scatter's flush
+    int fft_phased = 0;
     double* d_sn[3] = {nullptr, nullptr, nullptr};
 
     int64_t M = 0;
@@ -528,6 +532,14 @@ FftColGeom col_geom(const FftDev& f, int d, bool adjoint, int32_t nimg) {
     g.nk = f.nk;
     g.nouter = (int64_t)g.A * nimg;
     g.mult = nullptr;
+    g.pb = 0;
+    g.ortho = 0;
+    g.pb_ndim = f.ndim;
+    g.k0_mask = (int)f.K[0] - 1;
+    g.k0_shift = 0;
+    while (((int64_t)1 << g.k0_shift) < f.K[0]) ++g.k0_shift;
+    g.pbc[0] = g.pbc[1] = g.pbc[2] = 0.0;
+    g.scale_ortho = 1.0;
     return g;
 }
 
@@ -536,10 +548,17 @@ FftColGeom col_geom(const FftDev& f, int d, bool adjoint, int32_t nimg) {
 constexpr int64_t kFftTargetBlocks = 128;
 
 // Options of one pass: rezero (adjoint: leave the grid it reads all zero),
-// mult (adjoint: multiply the loaded spectrum by this grid-shaped array).
+// mult (adjoint: multiply the loaded spectrum by this grid-shaped array),
+// pb (column pass of an ndim >= 2 power-of-two grid: 1 multiplies the
+// output by phase_before and divides by scale_ortho if ortho, 2 multiplies
+// the input by conj(phase_before); pbc null means a zero phase).
 struct FftPassOpts {
     int rezero = 0;
     const void* mult = nullptr;
+    int pb = 0;
+    const double* pbc = nullptr;
+    int ortho = 0;
+    double scale_ortho = 1.0;
 };
 
 template <typename T, int K>
@@ -575,6 +594,12 @@ int launch_fft_cols(const FftDev& f, bool adjoint, int d, void* grid, int32_t ni
     FftColGeom g = col_geom(f, d, adjoint, nimg);
     g.zero_tail = o.rezero;
     g.mult = o.mult;
+    if (o.pb) {
+        g.pb = o.pb;
+        for (int e = 0; e < 3; ++e) g.pbc[e] = o.pbc ? o.pbc[e] : 0.0;
+        g.ortho = o.ortho;
+        g.scale_ortho = o.scale_ortho;
+    }
     // up to 128 contiguous bytes per row, while the pass keeps its workgroups
     int lpw = 1;
     while (2 * lpw * C::CXS <= 128 && g.S % (2 * lpw) == 0 && 2 * lpw * C::TL <= kFftMaxThreads &&
@@ -665,23 +690,39 @@ double adjoint_image_scale(const mi355nufft_plan* p) {
 
 // image -> oversampled grid: fftn(x * sn, Kd) (_nufft.py:1326-1331); the
 // phase_before of :1371 is applied by the gather
+// pb (fused passes, ndim >= 2 only): the last pass also applies
+// phase_before (real kernel) and the ortho scale, as the gather's staging
+// would.
 template <typename T>
-int forward_grid_impl(mi355nufft_plan* p, const void* x, const int64_t* xs, void* grid, int32_t nimg) {
+int forward_grid_impl(mi355nufft_plan* p, const void* x, const int64_t* xs, void* grid, int32_t nimg,
+                      int pb = 0) {
     if (!use_fused_fft(p)) {
         RET_IF(deapod_pad_impl<T>(p, x, xs, grid, nimg));
         return fft_impl(p, grid, nimg, -1);
     }
     const FftDev f = fft_dev(p);
     RET_IF(fft_rows<T>(f, false, x, nullptr, xs, grid, nimg, (T)1));
-    for (int d = 1; d < p->ndim; ++d) RET_IF(fft_cols<T>(f, false, d, grid, nimg));
+    for (int d = 1; d < p->ndim; ++d) {
+        FftPassOpts o;
+        if (pb && d == p->ndim - 1) {
+            o.pb = 1;
+            o.pbc = p->phasing == 0 ? p->phase_coef : nullptr;
+            o.ortho = p->ortho;
+            o.scale_ortho = p->scale_ortho;
+        }
+        RET_IF(fft_cols<T>(f, false, d, grid, nimg, o));
+    }
     return MI355NUFFT_OK;
 }
 
 // oversampled grid -> image: ifftn, crop to Nd, scale, * conj(sn)
 // (_nufft.py:1524-1572); the conj(phase_before) of :1519 is applied by the
 // scatter.  The grid is overwritten; with rezero it is left all zero.
+// pb (fused passes, real kernel, ndim >= 2 only): the first pass applies
+// conj(phase_before) instead of the scatter.
 template <typename T>
-int adjoint_image_impl(mi355nufft_plan* p, void* grid, void* x, const int64_t* xs, int32_t nimg, int rezero) {
+int adjoint_image_impl(mi355nufft_plan* p, void* grid, void* x, const int64_t* xs, int32_t nimg, int rezero,
+                       int pb = 0) {
     if (!use_fused_fft(p)) {
         RET_IF(fft_impl(p, grid, nimg, 1));
         return crop_deapod_impl<T>(p, grid, x, xs, nimg, rezero);
@@ -689,7 +730,14 @@ int adjoint_image_impl(mi355nufft_plan* p, void* grid, void* x, const int64_t* x
     const FftDev f = fft_dev(p);
     FftPassOpts o;
     o.rezero = rezero;
-    for (int d = p->ndim - 1; d >= 1; --d) RET_IF(fft_cols<T>(f, true, d, grid, nimg, o));
+    for (int d = p->ndim - 1; d >= 1; --d) {
+        FftPassOpts od = o;
+        if (pb && d == p->ndim - 1) {
+            od.pb = 2;
+            od.pbc = p->phase_coef;
+        }
+        RET_IF(fft_cols<T>(f, true, d, grid, nimg, od));
+    }
     return fft_rows<T>(f, true, nullptr, x, xs, grid, nimg, (T)adjoint_image_scale(p), o);
 }
 
@@ -981,6 +1029,7 @@ void setup_bin_geom(mi355nufft_plan* p) {
         }
         bg.nbins *= bg.nb[d];
         bg.tile *= bg.T[d];
+        bg.invT[d] = 1.0f / (float)bg.T[d];
     }
     bg.nbins_frame = bg.nbins;
     bg.nbins *= p->nframes;
@@ -1201,9 +1250,9 @@ int run_tiled(mi355nufft_plan* p, bool adjoint, const void* samples_in, void* sa
     a.ia.phase = p->has_phase ? (const cx<T>*)p->d_phase : nullptr;
     a.bg = p->bg;
     a.bs = bin_state(p);
-    a.use_pb = p->phasing == 0;
+    a.use_pb = p->phasing == 0 && !p->fft_phased;
     for (int d = 0; d < 3; ++d) a.pb[d] = (T)p->phase_coef[d];
-    a.ortho = adjoint ? 0 : p->ortho;
+    a.ortho = adjoint ? 0 : (p->ortho && !p->fft_phased);
     a.scale_ortho = (T)p->scale_ortho;
     a.tiles = adjoint ? (cx<T>*)p->tiles_target : nullptr;
     a.tms = (const T*)p->d_tms;
@@ -1459,6 +1508,24 @@ int toeplitz_apply_impl(mi355nufft_toeplitz* tp, const void* x, const int64_t* x
     }
     return fft_rows<T>(f, true, nullptr, y, ys, tp->d_grid, nimg, (T)1, o);
 }
+
+// Whether mi355nufft_forward / _adjoint move phase_before (and the forward's
+// ortho scale) into the fused FFT passes: power-of-two fused grids of two or
+// three dimensions, tiled kernels (atomic flush for the adjoint), table mode.
+bool fft_phase_forward(const mi355nufft_plan* p) {
+    return p->ndim >= 2 && use_fused_fft(p) && use_tiled(p) && !p->sp.on && (p->phasing == 0 || p->ortho);
+}
+bool fft_phase_adjoint(const mi355nufft_plan* p, int32_t nreps) {
+    return p->ndim >= 2 && p->phasing == 0 && use_fused_fft(p) && use_tiled(p) && !use_two_phase(p, nreps) &&
+           !p->sp.on;
+}
+
+// Sets fft_phased for its scope.
+struct FftPhased {
+    mi355nufft_plan* p;
+    FftPhased(mi355nufft_plan* q, bool on = true) : p(q) { p->fft_phased = on ? 1 : 0; }
+    ~FftPhased() { p->fft_phased = 0; }
+};
 
 int need_points(const mi355nufft_plan* p) {
     if (!p->d_tm && p->M > 0) return fail(MI355NUFFT_ESTATE, "points not set");
@@ -1914,6 +1981,14 @@ int mi355nufft_forward(mi355nufft_plan* p, const void* x, const int64_t* xs, voi
     const int32_t nimg = p->nframes * nreps;
     RET_IF(check_nreps(nimg));
     RET_IF(get_grid(p, nimg, &grid));
+    if (fft_phase_forward(p)) {
+        // the last FFT pass applies phase_before / the ortho scale; the
+        // gather stages the grid as it is
+        RET_IF(p->precision ? forward_grid_impl<double>(p, x, xs, grid, nimg, 1)
+                            : forward_grid_impl<float>(p, x, xs, grid, nimg, 1));
+        FftPhased on(p);
+        return p->precision ? interp_impl<double>(p, grid, y, nreps) : interp_impl<float>(p, grid, y, nreps);
+    }
     RET_IF(mi355nufft_forward_grid(p, x, xs, grid, nimg));
     return mi355nufft_interp(p, grid, y, nreps);
 }
@@ -1926,9 +2001,20 @@ int mi355nufft_adjoint(mi355nufft_plan* p, const void* y, void* x, const int64_t
     if (!x || !xs || (!y && p->M > 0)) return fail(MI355NUFFT_EINVAL, "y, x and xs are required");
     const int32_t nimg = p->nframes * nreps;
     RET_IF(check_nreps(nimg));
+    // phase: the first inverse FFT pass applies conj(phase_before) instead of
+    // the scatter's flush
+    const int phase = fft_phase_adjoint(p, nreps) ? 2 : 0;
     if (grid) {
-        RET_IF(mi355nufft_spread(p, y, grid, nreps));
-        return mi355nufft_adjoint_image(p, grid, x, xs, nimg);
+        if (!phase) {
+            RET_IF(mi355nufft_spread(p, y, grid, nreps));
+            return mi355nufft_adjoint_image(p, grid, x, xs, nimg);
+        }
+        {
+            FftPhased on(p);
+            RET_IF(p->precision ? spread_impl<double>(p, y, grid, nreps) : spread_impl<float>(p, y, grid, nreps));
+        }
+        return p->precision ? adjoint_image_impl<double>(p, grid, x, xs, nimg, 0, phase)
+                            : adjoint_image_impl<float>(p, grid, x, xs, nimg, 0, phase);
     }
     // plan-owned adjoint grid, kept zero between calls by the fused passes
     // (not under stream capture, where the host flag would run ahead of the
@@ -1940,11 +2026,15 @@ int mi355nufft_adjoint(mi355nufft_plan* p, const void* y, void* x, const int64_t
     p->agrid_zero = 0;
     RET_IF(need_points(p));
     if (!y && p->M > 0) return fail(MI355NUFFT_EINVAL, "y is required");
-    int rc = p->precision ? spread_impl<double>(p, y, p->d_agrid, nreps, was_zero)
+    int rc;
+    {
+        FftPhased on(p, phase != 0);
+        rc = p->precision ? spread_impl<double>(p, y, p->d_agrid, nreps, was_zero)
                           : spread_impl<float>(p, y, p->d_agrid, nreps, was_zero);
+    }
     RET_IF(rc);
-    rc = p->precision ? adjoint_image_impl<double>(p, p->d_agrid, x, xs, nimg, rezero)
-                      : adjoint_image_impl<float>(p, p->d_agrid, x, xs, nimg, rezero);
+    rc = p->precision ? adjoint_image_impl<double>(p, p->d_agrid, x, xs, nimg, rezero, phase)
+                      : adjoint_image_impl<float>(p, p->d_agrid, x, xs, nimg, rezero, phase);
     RET_IF(rc);
     if (!capturing) p->agrid_zero = rezero;
     return MI355NUFFT_OK;
