@@ -75,7 +75,10 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned n) {
     return x * q + (x < r ? x : r) + j;
 }
 
+// k mod K in [0, K).  Points with omega in [-pi, pi) give k in [-K, K),
+// handled without the (software) 64-bit division.
 __device__ __forceinline__ int pos_mod(long long k, int K) {
+    if (k >= -(long long)K && k < (long long)K) return (int)(k < 0 ? k + K : k);
     long long r = k % (long long)K;
     return (int)(r < 0 ? r + K : r);
 }
