@@ -254,31 +254,42 @@ __device__ __forceinline__ void load_sub(const BinGeom& bg, const BinState& bs, 
 // software.
 __device__ __forceinline__ int div_small(int e, float inv_t) { return (int)(((float)e + 0.5f) * inv_t); }
 
-// (o + e) mod K for 0 <= o < K, e >= 0
+// (o + e) mod K for 0 <= o < K, e >= 0: one conditional subtraction covers
+// tiles no wider than the grid plus one bin; wider tiles (tiny grids) take
+// the remainder.
 __device__ __forceinline__ int wrap_add(int o, int e, int K) {
     int g = o + e;
-    while (g >= K) g -= K;
+    g = g >= K ? g - K : g;
+    if (g >= K) g = g % K;
     return g;
 }
 
-// grid index of local tile point e (local coordinates e0, e1, e2)
+// grid index of local tile point e (local coordinates e0, e1, e2); axes at
+// or above NDIM are 0
+template <int NDIM = 3>
 __device__ __forceinline__ int64_t tile_to_grid(int e, const BinGeom& bg, const Sub& s, const AxisGeom* ax,
                                                 int& g0, int& g1, int& g2, int& e0, int& e1, int& e2) {
     const int q = div_small(e, bg.invT[0]);
     e0 = e - q * bg.T[0];
-    e2 = div_small(q, bg.invT[1]);
-    e1 = q - e2 * bg.T[1];
+    if (NDIM > 2) {
+        e2 = div_small(q, bg.invT[1]);
+        e1 = q - e2 * bg.T[1];
+    } else {
+        e2 = 0;
+        e1 = NDIM > 1 ? q : 0;
+    }
     g0 = wrap_add(s.o[0], e0, ax[0].K);
-    g1 = wrap_add(s.o[1], e1, ax[1].K);
-    g2 = wrap_add(s.o[2], e2, ax[2].K);
+    g1 = NDIM > 1 ? wrap_add(s.o[1], e1, ax[1].K) : 0;
+    g2 = NDIM > 2 ? wrap_add(s.o[2], e2, ax[2].K) : 0;
     return ((int64_t)g2 * ax[1].K + g1) * ax[0].K + g0;
 }
 
 // grid index of local tile point e
+template <int NDIM = 3>
 __device__ __forceinline__ int64_t tile_to_grid(int e, const BinGeom& bg, const Sub& s, const AxisGeom* ax,
                                                 int& g0, int& g1, int& g2) {
     int e0, e1, e2;
-    return tile_to_grid(e, bg, s, ax, g0, g1, g2, e0, e1, e2);
+    return tile_to_grid<NDIM>(e, bg, s, ax, g0, g1, g2, e0, e1, e2);
 }
 
 // Stage the tile of subproblem s from grid g into LDS: grid * phase_before
@@ -297,7 +308,7 @@ __device__ __forceinline__ void stage_tile(const TiledArgs<T, CK>& a, const Sub&
             const int e = e0 + u * blockDim.x;
             if (e < bg.tile) {
                 int g0, g1, g2;
-                const int64_t gi = tile_to_grid(e, bg, s, ax, g0, g1, g2);
+                const int64_t gi = tile_to_grid<NDIM>(e, bg, s, ax, g0, g1, g2);
                 v[u] = g[gi];
                 phi[u] = phase_before_arg<T>(c, NDIM, g0, g1, g2);
             }
@@ -563,7 +574,7 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
                 if (ar == 0.0 && ai == 0.0) continue;
                 T vr = (T)ar, vi = (T)ai;
                 int g0, g1, g2, e0, e1, e2;
-                const int64_t gi = tile_to_grid(e, bg, s, ax, g0, g1, g2, e0, e1, e2);
+                const int64_t gi = tile_to_grid<NDIM>(e, bg, s, ax, g0, g1, g2, e0, e1, e2);
                 if (a.use_pb) {  // * conj(phase_before)
                     const T c[3] = {a.pb[0], a.pb[1], a.pb[2]};
                     const cx<T> p = cis_tile(phase_before_arg<T>(c, NDIM, g0, g1, g2));
