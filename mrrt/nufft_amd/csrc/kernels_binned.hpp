@@ -429,22 +429,41 @@ __device__ __forceinline__ void scatter_taps(double* tre, double* tim, const Poi
                 atomicAdd(tim + pt.l[0] + j1, (double)vi);
             }
     } else if constexpr (NDIM == 2) {
+        // real kernel: the axis-0 coefficient times the row value in float64
+        // (the product of two floats is exact there), one conversion per
+        // coefficient and per row instead of two per tap
+        double c0d[JMAX];
+        if constexpr (!CK) {
+#pragma unroll
+            for (int j1 = 0; j1 < JMAX; ++j1) c0d[j1] = (double)pt.cr[0][j1];
+        }
 #pragma unroll
         for (int j2 = 0; j2 < JMAX; ++j2)
             if (FIXED || j2 < ax[1].J) {
                 T v2r, v2i;
                 cmul_conj<T, CK>(v2r, v2i, pt.cr[1][j2], pt.ci[1][j2], f.x, f.y);
                 const int row = (pt.l[1] + j2) * T0 + pt.l[0];
+                const double v2rd = (double)v2r, v2id = (double)v2i;
 #pragma unroll
                 for (int j1 = 0; j1 < JMAX; ++j1)
                     if (FIXED || j1 < ax[0].J) {
-                        T vr, vi;
-                        cmul_conj<T, CK>(vr, vi, pt.cr[0][j1], pt.ci[0][j1], v2r, v2i);
-                        atomicAdd(tre + row + j1, (double)vr);
-                        atomicAdd(tim + row + j1, (double)vi);
+                        if constexpr (CK) {
+                            T vr, vi;
+                            cmul_conj<T, CK>(vr, vi, pt.cr[0][j1], pt.ci[0][j1], v2r, v2i);
+                            atomicAdd(tre + row + j1, (double)vr);
+                            atomicAdd(tim + row + j1, (double)vi);
+                        } else {
+                            atomicAdd(tre + row + j1, c0d[j1] * v2rd);
+                            atomicAdd(tim + row + j1, c0d[j1] * v2id);
+                        }
                     }
             }
     } else {
+        double c0d[JMAX];
+        if constexpr (!CK) {
+#pragma unroll
+            for (int j1 = 0; j1 < JMAX; ++j1) c0d[j1] = (double)pt.cr[0][j1];
+        }
 #pragma unroll FIXED ? JMAX : 1
         for (int j3 = 0; j3 < JMAX; ++j3)
             if (FIXED || j3 < ax[2].J) {
@@ -456,13 +475,19 @@ __device__ __forceinline__ void scatter_taps(double* tre, double* tim, const Poi
                         T v2r, v2i;
                         cmul_conj<T, CK>(v2r, v2i, pt.cr[1][j2], pt.ci[1][j2], v3r, v3i);
                         const int row = ((pt.l[2] + j3) * T1 + (pt.l[1] + j2)) * T0 + pt.l[0];
+                        const double v2rd = (double)v2r, v2id = (double)v2i;
 #pragma unroll
                         for (int j1 = 0; j1 < JMAX; ++j1)
                             if (FIXED || j1 < ax[0].J) {
-                                T vr, vi;
-                                cmul_conj<T, CK>(vr, vi, pt.cr[0][j1], pt.ci[0][j1], v2r, v2i);
-                                atomicAdd(tre + row + j1, (double)vr);
-                                atomicAdd(tim + row + j1, (double)vi);
+                                if constexpr (CK) {
+                                    T vr, vi;
+                                    cmul_conj<T, CK>(vr, vi, pt.cr[0][j1], pt.ci[0][j1], v2r, v2i);
+                                    atomicAdd(tre + row + j1, (double)vr);
+                                    atomicAdd(tim + row + j1, (double)vi);
+                                } else {
+                                    atomicAdd(tre + row + j1, c0d[j1] * v2rd);
+                                    atomicAdd(tim + row + j1, c0d[j1] * v2id);
+                                }
                             }
                     }
             }
