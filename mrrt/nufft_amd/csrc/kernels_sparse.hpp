@@ -293,6 +293,20 @@ __global__ __launch_bounds__(256) void sp_spread_kernel(const SpPiece* __restric
     }
 }
 
+// out[r*M + m] = y[r*M + m] * conj(phase[m]) (:1497), once per sample so
+// the column pieces gather one value per entry
+template <typename T>
+__global__ __launch_bounds__(256) void sp_conj_phase_kernel(const cx<T>* __restrict__ y,
+                                                            const cx<T>* __restrict__ phase, int64_t M,
+                                                            cx<T>* __restrict__ out) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t r = blockIdx.y;
+    if (m >= M) return;
+    const cx<T> v = y[r * M + m];
+    const cx<T> ph = phase[m];
+    out[r * M + m] = mkcx<T>(v.x * ph.x + v.y * ph.y, v.y * ph.x - v.x * ph.y);
+}
+
 // Adds the partial sums of each split column in slot order.
 template <typename T>
 __global__ __launch_bounds__(256) void sp_combine_kernel(const SpSplit* __restrict__ splits, int64_t nsplit,

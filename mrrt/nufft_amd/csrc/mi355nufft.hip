@@ -81,6 +81,8 @@ struct SparseState {
     SpSplit* splits = nullptr;
     void* part = nullptr;      // partial sums of split columns (nslots x nreps)
     size_t part_bytes = 0;
+    void* ybuf = nullptr;      // samples times conj(phase_after) (M x nreps)
+    size_t ybuf_bytes = 0;
 };
 
 }  // namespace
@@ -744,7 +746,7 @@ int adjoint_image_impl(mi355nufft_plan* p, void* grid, void* x, const int64_t* x
 // ---------------------------------------------------------------- sparse mode
 void sparse_free(mi355nufft_plan* p) {
     SparseState& s = p->sp;
-    void* bufs[] = {s.val, s.col, s.perm, s.tval, s.trow, s.pieces, s.splits, s.part};
+    void* bufs[] = {s.val, s.col, s.perm, s.tval, s.trow, s.pieces, s.splits, s.part, s.ybuf};
     bool any = false;
     for (void* b : bufs) any = any || b;
     if (any) (void)hipStreamSynchronize(p->stream);
@@ -883,7 +885,8 @@ int sparse_build(mi355nufft_plan* p, const void* const* coef, const int32_t* con
 template <typename T>
 int sparse_interp(mi355nufft_plan* p, const void* grid, void* y, int32_t nreps, bool raw) {
     const SparseState& s = p->sp;
-    if (!raw && (p->phasing == 0 || p->ortho))
+    // (fft_phased: the forward's last FFT pass applied phase_before / ortho)
+    if (!raw && !p->fft_phased && (p->phasing == 0 || p->ortho))
         RET_IF(phase_grid_impl<T>(p, const_cast<void*>(grid), nreps, p->phasing == 0, 0, p->ortho));
     if (s.M == 0) return MI355NUFFT_OK;
     const cx<T>* ph = (!raw && p->phasing == 0 && p->has_phase) ? (const cx<T>*)p->d_phase : nullptr;
@@ -902,22 +905,37 @@ int sparse_interp(mi355nufft_plan* p, const void* grid, void* y, int32_t nreps, 
 // Adjoint with the sparse matrix; writes every grid point.  raw: grid =
 // P^H y exactly (grid_only, :1505-1509); else y is multiplied by
 // conj(phase_after) (:1497) and the grid by conj(phase_before) (:1519-1520).
+// grow a plan-owned sparse-mode buffer (stream-ordered free)
+int sparse_grow(mi355nufft_plan* p, void** buf, size_t* cap, size_t need, const char* what) {
+    if (need <= *cap) return MI355NUFFT_OK;
+    if (*buf) {
+        HIP_TRY(hipStreamSynchronize(p->stream));
+        (void)hipFree(*buf);
+        *buf = nullptr;
+        *cap = 0;
+    }
+    if (hipMalloc(buf, need ? need : 1) != hipSuccess) return fail(MI355NUFFT_ENOMEM, "%s", what);
+    *cap = need;
+    return MI355NUFFT_OK;
+}
+
 template <typename T>
 int sparse_spread(mi355nufft_plan* p, const void* y, void* grid, int32_t nreps, bool raw) {
     SparseState& s = p->sp;
-    const size_t need = (size_t)s.nslots * (size_t)nreps * p->cx_size();
-    if (need > s.part_bytes) {
-        if (s.part) {
-            HIP_TRY(hipStreamSynchronize(p->stream));
-            (void)hipFree(s.part);
-            s.part = nullptr;
-            s.part_bytes = 0;
-        }
-        if (hipMalloc(&s.part, need) != hipSuccess) return fail(MI355NUFFT_ENOMEM, "sparse partial sums");
-        s.part_bytes = need;
+    RET_IF(sparse_grow(p, &s.part, &s.part_bytes, (size_t)s.nslots * (size_t)nreps * p->cx_size(),
+                       "sparse partial sums"));
+    if (!raw && p->phasing == 0 && p->has_phase && s.M > 0) {
+        // y * conj(phase_after) once per sample
+        RET_IF(sparse_grow(p, &s.ybuf, &s.ybuf_bytes, (size_t)s.M * (size_t)nreps * p->cx_size(),
+                           "sparse sample buffer"));
+        hipLaunchKernelGGL((sp_conj_phase_kernel<T>), dim3(nblocks(s.M), nreps), dim3(kBlock), 0, p->stream,
+                           (const cx<T>*)y, (const cx<T>*)p->d_phase, s.M, (cx<T>*)s.ybuf);
+        LAUNCH_CHECK();
+        y = s.ybuf;
     }
-    const cx<T>* ph = (!raw && p->phasing == 0 && p->has_phase) ? (const cx<T>*)p->d_phase : nullptr;
-    const int with_pb = (!raw && p->phasing == 0) ? 1 : 0;
+    const cx<T>* ph = nullptr;
+    // (fft_phased: the adjoint's first FFT pass applies conj(phase_before))
+    const int with_pb = (!raw && p->phasing == 0 && !p->fft_phased) ? 1 : 0;
     const GridGeom g = grid_geom(p, nullptr);
     const T c0 = (T)p->phase_coef[0], c1 = (T)p->phase_coef[1], c2 = (T)p->phase_coef[2];
     const unsigned nb = (unsigned)((s.npieces * kSpLanes + kBlock - 1) / kBlock);
@@ -1511,13 +1529,14 @@ int toeplitz_apply_impl(mi355nufft_toeplitz* tp, const void* x, const int64_t* x
 
 // Whether mi355nufft_forward / _adjoint move phase_before (and the forward's
 // ortho scale) into the fused FFT passes: power-of-two fused grids of two or
-// three dimensions, tiled kernels (atomic flush for the adjoint), table mode.
+// three dimensions, with the sparse matrix or the tiled kernels (atomic
+// flush for the adjoint).
 bool fft_phase_forward(const mi355nufft_plan* p) {
-    return p->ndim >= 2 && use_fused_fft(p) && use_tiled(p) && !p->sp.on && (p->phasing == 0 || p->ortho);
+    return p->ndim >= 2 && use_fused_fft(p) && (p->sp.on || use_tiled(p)) && (p->phasing == 0 || p->ortho);
 }
 bool fft_phase_adjoint(const mi355nufft_plan* p, int32_t nreps) {
-    return p->ndim >= 2 && p->phasing == 0 && use_fused_fft(p) && use_tiled(p) && !use_two_phase(p, nreps) &&
-           !p->sp.on;
+    return p->ndim >= 2 && p->phasing == 0 && use_fused_fft(p) &&
+           (p->sp.on || (use_tiled(p) && !use_two_phase(p, nreps)));
 }
 
 // Sets fft_phased for its scope.
