@@ -137,6 +137,30 @@ def test_many_repetitions_vs_oracle(precision, phasing, Nd, Kd, nreps):
 
 
 @pytest.mark.parametrize("precision", ["single", "double"])
+@pytest.mark.parametrize("phasing", ["real", "complex"])
+@pytest.mark.parametrize("ortho", [False, True])
+@pytest.mark.parametrize("Nd,Kd", [((48, 40), (64, 128)), ((12, 10, 8), (32, 16, 16))])
+def test_fft_pass_phase_vs_oracle(precision, phasing, ortho, Nd, Kd):
+    """Power-of-two grids: the forward's last FFT pass applies phase_before
+    and the ortho scale, the adjoint's first pass conj(phase_before).  Both
+    against the oracle and against the rocFFT path of the same plan (where the
+    gather and the scatter apply them)."""
+    om = np.random.default_rng(21).uniform(-np.pi, np.pi, (900, len(Nd)))
+    A, B = _pair(Nd, om, Jd=6, Kd=Kd, Ld=512, n_shift=tuple(1.5 for _ in Nd), ortho=ortho, precision=precision,
+                 phasing=phasing)
+    assert A.plan_info()["fused_fft"]
+    x = trj.complex_gaussian(Nd + (2,), seed=22)
+    y = A.fft(x)
+    assert rel_l2(y, B.fft(x)) <= TOL[precision]
+    yb = trj.complex_gaussian((om.shape[0], 2), seed=23)
+    xa = A.adj(yb)
+    assert rel_l2(xa, B.adj(yb)) <= TOL[precision]
+    A.set_fft("rocfft")
+    assert rel_l2(A.fft(x), y) <= TOL[precision]
+    assert rel_l2(A.adj(yb), xa) <= TOL[precision]
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
 def test_operator_options_vs_oracle(precision):
     """ortho, adjoint_scalefactor, repetitions, order='C', real input."""
     Nd, Kd = (32, 32), (64, 64)

@@ -38,6 +38,9 @@ def _pair(Nd, om, **kw):
     ((32, 24), (64, 49), (6, 7), (3.3, -2.0), False),
     ((48, 40), (96, 80), 6, None, True),
     ((16, 12, 10), (32, 24, 20), (6, 5, 4), (1.0, 2.0, -3.0), False),
+    # power-of-two grids: phase_before in the fused FFT passes
+    ((48, 40), (64, 128), 6, (2.0, -1.0), True),
+    ((12, 10, 8), (32, 16, 16), 6, None, False),
 ])
 def test_sparse_operator_vs_oracle(precision, phasing, Nd, Kd, Jd, n_shift, ortho):
     if len(Nd) == 2:
