@@ -288,13 +288,15 @@ int mi355nufft_set_fft(mi355nufft_plan *plan, int method);
  * phase_after, ortho and the FFTs are unchanged.
  *   coef[d]  device (J_d, M) factors ud[d] of :783-802: float64 c (phasing
  *            0) or complex128 exp(i gam (N_d-1)/2 arg) * c (phasing 1)
- *   kidx[d]  device (J_d, M) int32 mod(koff + 1..J_d, K_d) (:788-791)
+ *   kidx[d]  device (J_d, M) int32 mod(koff + 1..J_d, K_d) (:788-791):
+ *            kidx[d][j*M + m] must equal (kidx[d][m] + j) mod K_d
  *   phase    device complex128 (M) exp(i omega . n_shift) of :833-838
  *            (phasing 1 with a nonzero n_shift), or NULL
  * Entry (m, kk) = prod_d ud[d] in float64 / complex128 (conjugated for
  * phasing 1, times phase), rounded to the plan's precision (:854-858).
- * The matrix is held twice (rows for the forward, columns for the
- * adjoint), 2 * M * prod(Jd) values and indices; the adjoint writes every
+ * The matrix is held twice: rows for the forward (M * prod(Jd) values and
+ * each row's first grid index per axis) and columns for the adjoint
+ * (M * prod(Jd) values and sample indices); the adjoint writes every
  * grid point once, without atomics, so it is bit-reproducible.  Needs
  * M * prod(Jd) < 2^31 and one frame.
  */

@@ -4,11 +4,14 @@
 //
 // The device holds P twice:
 //
-// * rows (forward): entry j of the row at sorted position i sits at
-//   [j*M + i] (value, Fortran flat grid index), j in the reference's tap
-//   order (axis 0 fastest, :813-823).  Rows follow the bin order of the
-//   points, so neighbouring threads gather neighbouring grid points, and the
-//   entry-major layout makes every value / index load coalesced.
+// * rows (forward): the value of entry j of the row at sorted position i
+//   sits at [j*M + i], j in the reference's tap order (axis 0 fastest,
+//   :813-823), and the row's first grid index per axis at [d*M + i].  The
+//   taps of axis d are (first + j_d) mod K_d (the kd of :791), so the grid
+//   index of every entry follows from the row's first taps and only the
+//   values are streamed.  Rows follow the bin order of the points, so
+//   neighbouring threads gather neighbouring grid points, and the
+//   entry-major layout makes every value load coalesced.
 // * columns (adjoint): the entries sorted by grid index (a stable sort, so
 //   within a column they keep the sorted-row order), cut into pieces of at
 //   most kSpPiece entries.  kSpLanes lanes sum one piece.  A column of one
@@ -65,12 +68,15 @@ __global__ __launch_bounds__(256) void sp_fill_rows_kernel(
     const void* __restrict__ coef0, const void* __restrict__ coef1, const void* __restrict__ coef2,
     const int32_t* __restrict__ kidx0, const int32_t* __restrict__ kidx1, const int32_t* __restrict__ kidx2,
     const double2* __restrict__ phase, const int32_t* __restrict__ perm, int64_t M, int ndim, int J0, int J1,
-    int J2, int K0, int K1, tab<T, CV>* __restrict__ val, int32_t* __restrict__ col, uint32_t* __restrict__ key,
+    int J2, int K0, int K1, tab<T, CV>* __restrict__ val, int32_t* __restrict__ rbase, uint32_t* __restrict__ key,
     uint32_t* __restrict__ pay) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
     const int64_t m = perm[i];
     const int nnzr = J0 * J1 * J2;
+    rbase[i] = kidx0[m];
+    if (ndim > 1) rbase[M + i] = kidx1[m];
+    if (ndim > 2) rbase[2 * M + i] = kidx2[m];
     int j = 0;
     for (int j2 = 0; j2 < J2; ++j2) {
         for (int j1 = 0; j1 < J1; ++j1) {
@@ -96,7 +102,6 @@ __global__ __launch_bounds__(256) void sp_fill_rows_kernel(
                     if (ndim > 2) re = re * ((const double*)coef2)[(int64_t)j2 * M + m];
                 }
                 store_val<T>(val, (int64_t)j * M + i, re, im);
-                col[(int64_t)j * M + i] = (int32_t)kk;
                 key[i * nnzr + j] = (uint32_t)kk;
                 pay[i * nnzr + j] = (uint32_t)(i * nnzr + j);
             }
@@ -209,29 +214,48 @@ __device__ __forceinline__ cx<T> sp_cmul(cx<T> a, cx<T> b) {
     return mkcx<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// (b + j) mod K for 0 <= b < K, 0 <= j < J
+__device__ __forceinline__ int sp_wrap(int b, int j, int K) {
+    int k = b + j;
+    k = k >= K ? k - K : k;
+    if (k >= K) k = k % K;  // only when K < J
+    return k;
+}
+
 // Forward, y = P xk: one thread per sorted row, up to 4 repetitions per pass
 // over the row.  phase (per sample, original order): phase_after, or null.
 template <typename T, bool CV>
 __global__ __launch_bounds__(256) void sp_interp_kernel(const tab<T, CV>* __restrict__ val,
-                                                        const int32_t* __restrict__ col,
-                                                        const int32_t* __restrict__ perm, int64_t M, int nnzr,
+                                                        const int32_t* __restrict__ rbase,
+                                                        const int32_t* __restrict__ perm, int64_t M, int ndim,
+                                                        int J0, int J1, int J2, int K0, int K1, int K2,
                                                         const cx<T>* __restrict__ grid, int64_t nk, int32_t nreps,
                                                         const cx<T>* __restrict__ phase, cx<T>* __restrict__ y) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
     const int64_t m = perm[i];
+    const int b0 = rbase[i];
+    const int b1 = ndim > 1 ? rbase[M + i] : 0;
+    const int b2 = ndim > 2 ? rbase[2 * M + i] : 0;
     for (int32_t r0 = 0; r0 < nreps; r0 += 4) {
         const int nr = nreps - r0 < 4 ? nreps - r0 : 4;
         cx<T> acc[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = mkcx<T>(0, 0);
         const cx<T>* g0 = grid + (int64_t)r0 * nk;
-        for (int j = 0; j < nnzr; ++j) {
-            const tab<T, CV> v = val[(int64_t)j * M + i];
-            const int64_t c = col[(int64_t)j * M + i];
+        int j = 0;
+        for (int j2 = 0; j2 < J2; ++j2) {
+            const int64_t c2 = ndim > 2 ? (int64_t)sp_wrap(b2, j2, K2) * K1 : 0;
+            for (int j1 = 0; j1 < J1; ++j1) {
+                const int64_t c12 = ((ndim > 1 ? sp_wrap(b1, j1, K1) : 0) + c2) * K0;
+                for (int j0 = 0; j0 < J0; ++j0, ++j) {
+                    const tab<T, CV> v = val[(int64_t)j * M + i];
+                    const int64_t c = c12 + sp_wrap(b0, j0, K0);
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (q < nr) sp_mac<T>(acc[q], v, g0[(int64_t)q * nk + c]);
+                    for (int q = 0; q < 4; ++q)
+                        if (q < nr) sp_mac<T>(acc[q], v, g0[(int64_t)q * nk + c]);
+                }
+            }
         }
         const cx<T> ph = phase ? phase[m] : mkcx<T>(1, 0);
 #pragma unroll

@@ -73,7 +73,7 @@ struct SparseState {
     int nnzr = 0;   // entries per row, prod J
     int64_t M = 0, nnz = 0, npieces = 0, nsplit = 0, nslots = 0;
     void* val = nullptr;       // (nnzr, M) real or complex, sorted rows
-    int32_t* col = nullptr;    // (nnzr, M) grid index
+    int32_t* rbase = nullptr;  // (ndim, M) first grid index per axis of each sorted row
     int32_t* perm = nullptr;   // sorted row -> sample
     void* tval = nullptr;      // (nnz) values in column order
     int32_t* trow = nullptr;   // (nnz) sample of each column entry
@@ -746,7 +746,7 @@ int adjoint_image_impl(mi355nufft_plan* p, void* grid, void* x, const int64_t* x
 // ---------------------------------------------------------------- sparse mode
 void sparse_free(mi355nufft_plan* p) {
     SparseState& s = p->sp;
-    void* bufs[] = {s.val, s.col, s.perm, s.tval, s.trow, s.pieces, s.splits, s.part, s.ybuf};
+    void* bufs[] = {s.val, s.rbase, s.perm, s.tval, s.trow, s.pieces, s.splits, s.part, s.ybuf};
     bool any = false;
     for (void* b : bufs) any = any || b;
     if (any) (void)hipStreamSynchronize(p->stream);
@@ -792,7 +792,7 @@ int sparse_build(mi355nufft_plan* p, const void* const* coef, const int32_t* con
     s.M = M;
     s.nnz = nnz;
     auto alloc = [&](void** q, size_t bytes) { return hipMalloc(q, bytes ? bytes : 1) == hipSuccess; };
-    if (!alloc(&s.val, (size_t)nnz * sizeof(V)) || !alloc((void**)&s.col, (size_t)nnz * 4) ||
+    if (!alloc(&s.val, (size_t)nnz * sizeof(V)) || !alloc((void**)&s.rbase, (size_t)M * p->ndim * 4) ||
         !alloc((void**)&s.perm, (size_t)(M ? M : 1) * 4) || !alloc(&s.tval, (size_t)nnz * sizeof(V)) ||
         !alloc((void**)&s.trow, (size_t)nnz * 4)) {
         sparse_free(p);
@@ -822,7 +822,7 @@ int sparse_build(mi355nufft_plan* p, const void* const* coef, const int32_t* con
                            p->ndim > 1 ? coef[1] : nullptr, p->ndim > 2 ? coef[2] : nullptr, kidx[0],
                            p->ndim > 1 ? kidx[1] : nullptr, p->ndim > 2 ? kidx[2] : nullptr, (const double2*)phase,
                            s.perm, M, p->ndim, p->J[0], J1, J2, (int)p->K[0], (int)(p->ndim > 1 ? p->K[1] : 1),
-                           (V*)s.val, s.col, key[0], pay[0]);
+                           (V*)s.val, s.rbase, key[0], pay[0]);
         LAUNCH_CHECK();
         int bits = 1;
         while (((int64_t)1 << bits) < nk) ++bits;
@@ -890,14 +890,16 @@ int sparse_interp(mi355nufft_plan* p, const void* grid, void* y, int32_t nreps, 
         RET_IF(phase_grid_impl<T>(p, const_cast<void*>(grid), nreps, p->phasing == 0, 0, p->ortho));
     if (s.M == 0) return MI355NUFFT_OK;
     const cx<T>* ph = (!raw && p->phasing == 0 && p->has_phase) ? (const cx<T>*)p->d_phase : nullptr;
+    const int J1 = p->ndim > 1 ? p->J[1] : 1, J2 = p->ndim > 2 ? p->J[2] : 1;
+    const int K1 = (int)(p->ndim > 1 ? p->K[1] : 1), K2 = (int)(p->ndim > 2 ? p->K[2] : 1);
     if (s.cvals)
         hipLaunchKernelGGL((sp_interp_kernel<T, true>), dim3(nblocks(s.M)), dim3(kBlock), 0, p->stream,
-                           (const cx<T>*)s.val, s.col, s.perm, s.M, s.nnzr, (const cx<T>*)grid, p->nk, nreps, ph,
-                           (cx<T>*)y);
+                           (const cx<T>*)s.val, s.rbase, s.perm, s.M, p->ndim, p->J[0], J1, J2, (int)p->K[0], K1,
+                           K2, (const cx<T>*)grid, p->nk, nreps, ph, (cx<T>*)y);
     else
         hipLaunchKernelGGL((sp_interp_kernel<T, false>), dim3(nblocks(s.M)), dim3(kBlock), 0, p->stream,
-                           (const T*)s.val, s.col, s.perm, s.M, s.nnzr, (const cx<T>*)grid, p->nk, nreps, ph,
-                           (cx<T>*)y);
+                           (const T*)s.val, s.rbase, s.perm, s.M, p->ndim, p->J[0], J1, J2, (int)p->K[0], K1,
+                           K2, (const cx<T>*)grid, p->nk, nreps, ph, (cx<T>*)y);
     LAUNCH_CHECK();
     return MI355NUFFT_OK;
 }
