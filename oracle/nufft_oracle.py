@@ -36,6 +36,11 @@ interp_forward/adj     reference C kernels (oracle/_ref) on golden vectors;
 dtft / dtft_adj        FFT at uniform frequencies (test_dtft.py) atol 1e-12
 NufftOracle.fft/adj    composition of the above + numpy.fft; vs dtft at the
                        reference's own tolerances (test_nufft.py:60-324)
+sparse_matrix          every entry vs a brute-force sum over the whole grid
+                       (1-D/2-D/3-D, real and complex kernel, n_shift); the
+                       sparse-mode operator vs dtft.  Sparse mode's
+                       phase_after from the cast omega (_nufft.py:768-769) is
+                       pinned only by reading the source
 =====================  ====================================================
 """
 
