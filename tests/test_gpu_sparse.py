@@ -251,3 +251,19 @@ def test_sparse_nufft_3d_vs_dtft(precision, phasing, order):
     x_adj = A.adj(y)
     np.testing.assert_allclose(x_adj, O.dtft_adj(y, om, tuple(Nd), n_shift).reshape(Nd, order="F"),
                                rtol=1e-2, atol=1e-4)
+
+
+@pytest.mark.parametrize("mode", ["table", "sparse"])
+def test_empty_points(mode):
+    """No sample points: the forward returns no samples; the device adjoint
+    returns a zero image.  The array-level adjoint raises like the
+    reference's size check (xk.size % obj.M, _nufft.py:1483)."""
+    Nd = (16, 12)
+    A = NufftBase(Nd, np.zeros((0, 2)), Jd=6, Kd=(32, 24), mode=mode, device=dev())
+    assert A.M == 0
+    y = A.fft(trj.complex_gaussian(Nd, seed=50))
+    assert y.shape == (0,)
+    xa = A._adjoint_dev(torch.zeros((1, 0), dtype=A._tcdt, device=dev()), 1)
+    assert tuple(xa.shape) == Nd + (1,) and not torch.any(xa != 0)
+    with pytest.raises(ZeroDivisionError):
+        A.adj(np.zeros((0,), np.complex64))
