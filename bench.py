@@ -28,6 +28,10 @@ sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
 
+# float64 LDS atomic adds per second over the chip: 64 lanes per 9.06 CU
+# cycles (measured for ds_add_f64 without bank conflicts), 256 CUs, 2.4 GHz
+LDS_F64_ATOMIC_PEAK_LANES_PER_S = 64 / 9.06 * 256 * 2.4e9
+
 KERNEL_NAMES = {"interp": "interp_tiled_kernel<float, 2, false, 6, true>",
                 "spread": "spread_tiled_kernel<float, 2, false, 6, true>"}
 
@@ -410,6 +414,18 @@ def main():
                 "bytes_per_launch": b,
                 "other_kernel": {"kernel": KERNEL_NAMES["interp" if dom == "spread" else "spread"],
                                  "kernel_ms": round(kern["interp" if dom == "spread" else "spread"][1], 5)}}
+    # the scatter's LDS-atomic view: M * J^d complex accumulations per launch,
+    # two float64 LDS atomic adds each, against the LDS atomic peak
+    acc = run.M * 6 ** 2
+    t_sc = kern["spread"][1] * 1e-3
+    lane_peak = LDS_F64_ATOMIC_PEAK_LANES_PER_S
+    roofline["scatter_lds_atomics"] = {
+        "accumulations_per_s": round(acc / t_sc, 1),
+        "lane_atomics_per_s": round(2 * acc / t_sc, 1),
+        "peak_lane_atomics_per_s": round(lane_peak, 1),
+        "frac": round(2 * acc / t_sc / lane_peak, 4),
+        "peak_basis": "ds_add_f64: 64 lanes per 9.06 CU-cycles (measured, conflict-free), 256 CUs, 2.4 GHz",
+    }
     extra = {}
     if not args.no_extra:
         other = "rocfft" if fft_choice == "auto" else "auto"
