@@ -307,15 +307,16 @@ def test_nufft_3d_vs_dtft(precision, phasing, order):
                                rtol=1e-2, atol=1e-4)
 
 
+@pytest.mark.parametrize("mode", ["table", "sparse"])
 @pytest.mark.parametrize("precision", ["single", "double"])
-def test_nufft_dtypes(precision):
-    """mrrt/nufft/tests/test_nufft.py:327-388."""
+def test_nufft_dtypes(precision, mode):
+    """mrrt/nufft/tests/test_nufft.py:327-388 (in both modes)."""
     Nd, Kd, Jd, Ld = 64, 128, 6, 4096
     om = perturbed_gridpoints(Nd)
-    A = NufftBase(omega=om, Nd=Nd, Jd=Jd, Kd=Kd, n_shift=Nd // 2, mode="table", Ld=Ld, precision=precision,
+    A = NufftBase(omega=om, Nd=Nd, Jd=Jd, Kd=Kd, n_shift=Nd // 2, mode=mode, Ld=Ld, precision=precision,
                   device=dev())
     cdt = np.complex64 if precision == "single" else np.complex128
-    A2 = NufftBase(omega=om.astype(A._real_dtype), Nd=Nd, Jd=Jd, Kd=Kd, n_shift=Nd // 2, mode="table", Ld=Ld,
+    A2 = NufftBase(omega=om.astype(A._real_dtype), Nd=Nd, Jd=Jd, Kd=Kd, n_shift=Nd // 2, mode=mode, Ld=Ld,
                    precision="auto", device=dev())
     assert A2._cplx_dtype == cdt
     x = np.random.randn(Nd) + 1j * np.random.randn(Nd)
@@ -324,11 +325,15 @@ def test_nufft_dtypes(precision):
 
 
 # ------------------------------------------------------------ properties
+@pytest.mark.parametrize("mode", ["table", "sparse"])
+@pytest.mark.parametrize("Kd", [(96, 80), (64, 64)])
 @pytest.mark.parametrize("precision", ["single", "double"])
-def test_adjointness(precision):
-    Nd, Kd = (48, 40), (96, 80)
+def test_adjointness(precision, Kd, mode):
+    """<A x, y> = <x, A^H y> to rounding; (64, 64) runs the fused FFT passes
+    with phase_before in them, (96, 80) rocFFT."""
+    Nd = (48, 40)
     om = trj.golden_angle_2d(30, 96)
-    A = NufftBase(Nd, om, Jd=6, Kd=Kd, Ld=1024, precision=precision, device=dev())
+    A = NufftBase(Nd, om, Jd=6, Kd=Kd, Ld=1024, precision=precision, mode=mode, device=dev())
     x = trj.complex_gaussian(Nd, seed=11)
     y = trj.complex_gaussian((om.shape[0],), seed=12)
     lhs = np.vdot(y, A.fft(x))
