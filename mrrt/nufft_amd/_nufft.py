@@ -497,14 +497,17 @@ def _numel(x):
 
 
 def _to_device(obj, x):
-    """Returns (tensor on device with the operator's complex dtype, was_numpy)."""
+    """Returns (tensor on device with the operator's complex dtype, was_numpy).
+
+    Host arrays go through a pinned staging copy (torch's caching host
+    allocator), so the transfer to the device runs at pinned-memory rate."""
     was_numpy = not isinstance(x, torch.Tensor)
     if was_numpy:
         x = np.asarray(x)
         if x.dtype == np.complex128 and obj._precision == "single":
             x = x.astype(np.complex64)
         t = torch.from_numpy(x) if x.dtype.kind in "fc" else torch.from_numpy(x.astype(np.float64))
-        t = t.to(obj.device)
+        t = t.pin_memory().to(obj.device, non_blocking=True)
     else:
         t = x.to(obj.device)
     if not t.is_complex():
@@ -516,8 +519,12 @@ def _to_device(obj, x):
 
 
 def _to_output(t, was_numpy):
+    """Host results are copied into a new pinned tensor (synchronous copy)
+    whose memory the returned array keeps alive."""
     if was_numpy:
-        return t.cpu().numpy()
+        out = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+        out.copy_(t)
+        return out.numpy()
     return t
 
 
