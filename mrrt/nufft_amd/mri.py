@@ -23,7 +23,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._nufft import NufftBase
+from ._nufft import NufftBase, _to_output
 from .batched import NufftFrames
 
 __all__ = ["SenseOperator"]
@@ -119,14 +119,17 @@ class SenseOperator:
 
     def _in(self, x, shape):
         was_numpy = not isinstance(x, torch.Tensor)
-        t = torch.as_tensor(np.asarray(x) if was_numpy else x).to(self.device)
+        if was_numpy:  # pinned staging copy, then an asynchronous transfer
+            t = torch.from_numpy(np.ascontiguousarray(x)).pin_memory().to(self.device, non_blocking=True)
+        else:
+            t = x.to(self.device)
         if tuple(t.shape) != tuple(shape):
             raise ValueError("expected shape {}, got {}".format(tuple(shape), tuple(t.shape)))
         return t.to(self._tcdt).contiguous(), was_numpy
 
     @staticmethod
     def _out(t, was_numpy):
-        return t.cpu().numpy() if was_numpy else t
+        return _to_output(t, was_numpy)
 
     # ------------------------------------------------------------ device API
     def forward_dev(self, x):

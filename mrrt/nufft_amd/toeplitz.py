@@ -22,7 +22,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._nufft import NufftBase
+from ._nufft import NufftBase, _to_output
 
 __all__ = ["ToeplitzNormal"]
 
@@ -124,7 +124,10 @@ class ToeplitzNormal:
         """A^H W A x for x of shape Nd or Nd + (nreps,), numpy or tensor;
         numpy in, numpy out."""
         was_numpy = not isinstance(x, torch.Tensor)
-        xt = torch.as_tensor(np.asarray(x) if was_numpy else x, device=self.device)
+        if was_numpy:  # pinned staging copy, then an asynchronous transfer
+            xt = torch.from_numpy(np.ascontiguousarray(x)).pin_memory().to(self.device, non_blocking=True)
+        else:
+            xt = x.to(self.device)
         squeeze = tuple(xt.shape) == self.Nd
         if squeeze:
             xt = xt.unsqueeze(-1)
@@ -133,7 +136,7 @@ class ToeplitzNormal:
         y = self.apply_dev(xt)
         if squeeze:
             y = y[..., 0]
-        return y.cpu().numpy() if was_numpy else y
+        return _to_output(y, was_numpy)
 
     def __del__(self):
         plan = getattr(self, "_plan", None)
