@@ -26,8 +26,14 @@
 
 namespace mi355 {
 
-constexpr int kSpLanes = 16;  // lanes summing one column piece
-constexpr int kSpPiece = 64;  // entries per column piece
+#ifndef MI355_SP_LANES
+#define MI355_SP_LANES 16
+#endif
+#ifndef MI355_SP_PIECE
+#define MI355_SP_PIECE 64
+#endif
+constexpr int kSpLanes = MI355_SP_LANES;  // lanes summing one column piece (power of two, <= 64)
+constexpr int kSpPiece = MI355_SP_PIECE;  // entries per column piece
 
 struct SpPiece {
     int32_t k;      // grid point (column of P)
