@@ -174,10 +174,12 @@ class GpuRun:
         return out
 
 
-def run_c4(steps=3, autotune=True):
+def run_c4(steps=3, autotune=True, binning=None, loop=True):
     """24 frames x 32 coils, golden-angle frames with 13..89 spokes x 512,
     Nd 256^2, Kd 512^2, J 6, complex64: one graph-captured frames plan vs a
-    per-frame loop of plain operators (32 coils each)."""
+    per-frame loop of plain operators (32 coils each).  ``binning`` =
+    (bin_size, max_points, threads) fixes the binning instead of tuning it;
+    ``loop=False`` skips the per-frame comparison (e.g. for profiling)."""
     import torch
     from mrrt.nufft_amd import NufftBase
     from mrrt.nufft_amd import trajectories as trj
@@ -187,7 +189,10 @@ def run_c4(steps=3, autotune=True):
     frames = trj.golden_angle_frames(F, 512, 13, 89, seed=0)
     B = NufftFrames(Nd, frames, C, Jd=6, Kd=Kd, Ld=1024, device=dev)
     tuned = None
-    if autotune:
+    if binning is not None:
+        B.op.set_binning(binning[0], binning[1], "auto", binning[2])
+        tuned = {"bin_size": binning[0], "max_points": binning[1], "threads": binning[2]}
+    elif autotune:
         from mrrt.nufft_amd import autotune as at
         b, pmax, _ = at.tune(B.op, nreps=C, use_cache=False)
         info = B.op.plan_info()
@@ -202,16 +207,17 @@ def run_c4(steps=3, autotune=True):
         B.replay_forward()
         B.replay_adjoint()
 
-    ops = [NufftBase(Nd, om, Jd=6, Kd=Kd, Ld=1024, device=dev, order="C") for om in frames]
+    ops = [NufftBase(Nd, om, Jd=6, Kd=Kd, Ld=1024, device=dev, order="C") for om in frames] if loop else []
 
-    def loop():
+    def loop_frames():
         for f, op in enumerate(ops):
             xv = x[f].permute(1, 2, 0)
             yf = op._forward_dev(xv, C)
             op._adjoint_dev(yf, C)
 
+    do_loop = loop
     t_b = _time_fn(batched, steps, 1)
-    t_l = _time_fn(loop, steps, 1)
+    t_l = _time_fn(loop_frames, steps, 1) if do_loop else float("nan")
     # stage split of one batched forward + adjoint (plan stages, 768 images)
     import ctypes
     from mrrt.nufft_amd import _lib

@@ -174,6 +174,8 @@ struct FftRowGeom {
     int64_t nlines;    // N1 * N2 * nimg
     int lpw;           // lines per workgroup (blockDim = lpw * TL)
     const void* mult;  // adjoint: complex T grid (one image) multiplying the loaded rows, or null
+    int across;        // image side read / written with lanes across neighbouring lines (image axis 1
+                       // is the contiguous one, as for C-ordered images), else along each line
 };
 
 template <typename T, int K>
@@ -210,15 +212,26 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_rows_pad_kernel(const cx<T
     if (live) row_of_line(line, g, n2, n3, r);
     cx<T>* ln = reinterpret_cast<cx<T>*>(smem) + li * C::KP;
     const double* sn[3] = {sn0, sn1, sn2};
-    for (int t = tl; t < K; t += TL) {
+    // image loads: along the line, or across neighbouring lines (consecutive
+    // threads on consecutive lines, which are adjacent in memory when image
+    // axis 1 is the contiguous one)
+    const int lj = g.across ? (int)(threadIdx.x % (unsigned)g.lpw) : li;
+    const int t0 = g.across ? (int)(threadIdx.x / (unsigned)g.lpw) : tl;
+    int m2 = n2, m3 = n3;
+    int64_t mr = r;
+    const int64_t mline = (int64_t)blockIdx.x * g.lpw + lj;
+    const bool mlive = mline < g.nlines;
+    if (g.across && mlive) row_of_line(mline, g, m2, m3, mr);
+    cx<T>* lm = reinterpret_cast<cx<T>*>(smem) + lj * C::KP;
+    for (int t = t0; t < K; t += TL) {
         cx<T> o = mkcx<T>((T)0, (T)0);
-        if (live && t < g.N[0]) {
-            const T s = sn0 ? deapod_factor<T>(sn, ndim, t, n2, n3) : (T)1;
-            const cx<T> v = x[(int64_t)t * g.xs[0] + (int64_t)n2 * g.xs[1] + (int64_t)n3 * g.xs[2] + r * g.xs[3]];
+        if (mlive && t < g.N[0]) {
+            const T s = sn0 ? deapod_factor<T>(sn, ndim, t, m2, m3) : (T)1;
+            const cx<T> v = x[(int64_t)t * g.xs[0] + (int64_t)m2 * g.xs[1] + (int64_t)m3 * g.xs[2] + mr * g.xs[3]];
             o.x = v.x * s;
             o.y = v.y * s;
         }
-        ln[pad(t)] = o;
+        lm[pad(t)] = o;
     }
     cx<T>* tws = stage_twiddles<T, K>(smem, g.lpw, tw);  // after the data loads are in flight
     __syncthreads();
@@ -261,15 +274,28 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_rows_crop_kernel(cx<T>* __
     if (rezero && live)  // each thread clears the points it loaded
         for (int t = tl; t < K; t += TL) in[t] = mkcx<T>((T)0, (T)0);
     fft_line<T, K, TL, SIGN>(ln, tl, tws);
-    if (live) {
+    // image stores: along the line, or across neighbouring lines (see
+    // fft_rows_pad_kernel), which needs the whole block's lines done
+    const int lj = g.across ? (int)(threadIdx.x % (unsigned)g.lpw) : li;
+    const int t0 = g.across ? (int)(threadIdx.x / (unsigned)g.lpw) : tl;
+    int m2 = n2, m3 = n3;
+    int64_t mr = r;
+    const int64_t mline = (int64_t)blockIdx.x * g.lpw + lj;
+    const bool mlive = mline < g.nlines;
+    if (g.across) {
+        __syncthreads();
+        if (mlive) row_of_line(mline, g, m2, m3, mr);
+    }
+    const cx<T>* lm = reinterpret_cast<const cx<T>*>(smem) + lj * C::KP;
+    if (mlive) {
         const double* sn[3] = {sn0, sn1, sn2};
-        for (int t = tl; t < g.N[0]; t += TL) {
-            const T s = sn0 ? deapod_factor<T>(sn, ndim, t, n2, n3) : (T)1;
-            const cx<T> v = ln[pad(t)];
+        for (int t = t0; t < g.N[0]; t += TL) {
+            const T s = sn0 ? deapod_factor<T>(sn, ndim, t, m2, m3) : (T)1;
+            const cx<T> v = lm[pad(t)];
             cx<T> o;
             o.x = (v.x * scale) * s;
             o.y = (v.y * scale) * s;
-            x[(int64_t)t * g.xs[0] + (int64_t)n2 * g.xs[1] + (int64_t)n3 * g.xs[2] + r * g.xs[3]] = o;
+            x[(int64_t)t * g.xs[0] + (int64_t)m2 * g.xs[1] + (int64_t)m3 * g.xs[2] + mr * g.xs[3]] = o;
         }
     }
 }

@@ -518,6 +518,9 @@ FftRowGeom row_geom(const FftDev& f, const int64_t* xs, int32_t nimg) {
     g.xs[3] = xs[f.ndim];
     g.nlines = (int64_t)g.N[1] * g.N[2] * nimg;
     g.mult = nullptr;
+    // C-ordered images (axis 1 contiguous): neighbouring lines are adjacent
+    // in memory, so the image side is read / written across lines
+    g.across = (f.ndim >= 2 && g.xs[1] == 1 && g.xs[0] != 1) ? 1 : 0;
     return g;
 }
 
@@ -569,8 +572,11 @@ int launch_fft_rows(const FftDev& f, bool adjoint, const void* x, void* xo, cons
     using C = FftCfg<T, K>;
     FftRowGeom g = row_geom(f, xs, nimg);
     g.mult = o.mult;
+    // more lines per workgroup across C-ordered images, for longer contiguous
+    // runs of the image side
+    const int max_threads = g.across ? 512 : 256;
     int lpw = 1;
-    while (2 * lpw * C::TL <= 256 && C::lds_bytes(2 * lpw) <= (size_t)kFftLds &&
+    while (2 * lpw * C::TL <= max_threads && C::lds_bytes(2 * lpw) <= (size_t)kFftLds &&
            g.nlines / (2 * lpw) >= kFftTargetBlocks)
         lpw *= 2;
     g.lpw = lpw;
