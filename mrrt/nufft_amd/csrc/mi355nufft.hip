@@ -551,6 +551,9 @@ FftColGeom col_geom(const FftDev& f, int d, bool adjoint, int32_t nimg) {
 // Workgroups a pass aims for (measured best on 512^2 grids): lines per workgroup grow only
 // while the pass keeps at least this many workgroups.
 constexpr int64_t kFftTargetBlocks = 128;
+// LDS cap of the row passes over C-ordered images (room for 16 lines of 512
+// complex64 points)
+constexpr size_t kFftLdsAcross = 81920;
 
 // Options of one pass: rezero (adjoint: leave the grid it reads all zero),
 // mult (adjoint: multiply the loaded spectrum by this grid-shaped array),
@@ -573,10 +576,11 @@ int launch_fft_rows(const FftDev& f, bool adjoint, const void* x, void* xo, cons
     FftRowGeom g = row_geom(f, xs, nimg);
     g.mult = o.mult;
     // more lines per workgroup across C-ordered images, for longer contiguous
-    // runs of the image side
-    const int max_threads = g.across ? 512 : 256;
+    // runs of the image side (16 lines of complex64: one 128-byte line)
+    const int max_threads = g.across ? 1024 : 256;
+    const size_t lds_cap = g.across ? kFftLdsAcross : (size_t)kFftLds;
     int lpw = 1;
-    while (2 * lpw * C::TL <= max_threads && C::lds_bytes(2 * lpw) <= (size_t)kFftLds &&
+    while (2 * lpw * C::TL <= max_threads && C::lds_bytes(2 * lpw) <= lds_cap &&
            g.nlines / (2 * lpw) >= kFftTargetBlocks)
         lpw *= 2;
     g.lpw = lpw;

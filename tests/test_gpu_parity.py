@@ -189,6 +189,22 @@ def test_operator_options_vs_oracle(precision):
 
 
 @pytest.mark.parametrize("precision", ["single", "double"])
+def test_many_lines_c_order_vs_oracle(precision):
+    """Repetitions leading over C-ordered images with K0 = 512 and 2048 rows:
+    the axis-0 FFT passes take up to 16 neighbouring rows per workgroup."""
+    Nd, Kd, R = (256, 16), (512, 32), 128
+    om = np.random.default_rng(21).uniform(-np.pi, np.pi, (1500, 2))
+    A = NufftBase(Nd, om, Jd=6, Kd=Kd, Ld=1024, precision=precision, order="C", device=dev())
+    B = O.NufftOracle(Nd, om, Jd=6, Kd=Kd, Ld=1024, precision=precision)
+    x = trj.complex_gaussian((R,) + Nd, seed=22)
+    y = A.fft(x)
+    assert y.shape == (R, om.shape[0])
+    assert rel_l2(y.T, B.fft(np.moveaxis(x, 0, -1))) <= TOL[precision]
+    xa = A.adj(y)
+    assert rel_l2(np.moveaxis(xa, 0, -1), B.adj(y.T.astype(B.cdt))) <= TOL[precision]
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
 def test_c1_radial_vs_oracle_and_nudft(precision):
     """Nd 128^2, Kd 256^2, J 6, L 1024, 64 radial spokes x 256 readout."""
     Nd, Kd = (128, 128), (256, 256)
