@@ -2047,14 +2047,16 @@ int mi355nufft_adjoint(mi355nufft_plan* p, const void* y, void* x, const int64_t
         return p->precision ? adjoint_image_impl<double>(p, grid, x, xs, nimg, 0, phase)
                             : adjoint_image_impl<float>(p, grid, x, xs, nimg, 0, phase);
     }
-    // plan-owned adjoint grid, kept zero between calls by the fused passes
-    // (not under stream capture, where the host flag would run ahead of the
-    // captured work)
+    // Plan-owned adjoint grid, left all zero by every adjoint that completes
+    // (the passes write zeros over what they read), so the scatter needs no
+    // clearing pass when the flag says so.  A captured adjoint ends the same
+    // way on every replay: it skips the clearing pass when the grid is zero
+    // at capture, and capture leaves the flag as it was.
     RET_IF(get_agrid(p, nimg));
     const bool capturing = stream_capturing(p);
-    const bool was_zero = p->agrid_zero && !capturing;
+    const bool was_zero = p->agrid_zero != 0;
     const int rezero = 1;
-    p->agrid_zero = 0;
+    if (!capturing) p->agrid_zero = 0;
     RET_IF(need_points(p));
     if (!y && p->M > 0) return fail(MI355NUFFT_EINVAL, "y is required");
     int rc;
@@ -2063,10 +2065,16 @@ int mi355nufft_adjoint(mi355nufft_plan* p, const void* y, void* x, const int64_t
         rc = p->precision ? spread_impl<double>(p, y, p->d_agrid, nreps, was_zero)
                           : spread_impl<float>(p, y, p->d_agrid, nreps, was_zero);
     }
-    RET_IF(rc);
-    rc = p->precision ? adjoint_image_impl<double>(p, p->d_agrid, x, xs, nimg, rezero, phase)
-                      : adjoint_image_impl<float>(p, p->d_agrid, x, xs, nimg, rezero, phase);
-    RET_IF(rc);
+    if (rc == MI355NUFFT_OK)
+        rc = p->precision ? adjoint_image_impl<double>(p, p->d_agrid, x, xs, nimg, rezero, phase)
+                          : adjoint_image_impl<float>(p, p->d_agrid, x, xs, nimg, rezero, phase);
+    if (rc != MI355NUFFT_OK) {
+        // clear what a partial call left, for adjoints captured earlier
+        if (!capturing)
+            (void)(p->precision ? zero_grid<double>(p->d_agrid, (int64_t)p->nframes * nreps * p->nk, p->stream)
+                                : zero_grid<float>(p->d_agrid, (int64_t)p->nframes * nreps * p->nk, p->stream));
+        return rc;
+    }
     if (!capturing) p->agrid_zero = rezero;
     return MI355NUFFT_OK;
 }

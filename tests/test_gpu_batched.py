@@ -72,6 +72,10 @@ def test_frames_graph_replays_match_eager():
         B.replay_adjoint()
         torch.cuda.synchronize()
         assert rel_l2(xa.cpu().numpy(), xa_ref.cpu().numpy()) <= 1e-6
+        # an eager adjoint between replays shares the plan's adjoint grid,
+        # which both leave all zero
+        xe = B.adjoint(y_ref)
+        assert rel_l2(xe.cpu().numpy(), xa_ref.cpu().numpy()) <= 1e-6
         _ = (x * 2).sum().item()      # eager work between replays
 
 
