@@ -195,7 +195,11 @@ int mi355nufft_forward(mi355nufft_plan *plan, const void *x, const int64_t *xs,
  *              NULL to use a plan-owned buffer.  The last FFT stage writes
  *              zeros over the plan-owned buffer as it reads it, so each
  *              adjoint leaves it all zero and the next one skips clearing
- *              it (except under stream capture, where it is always cleared).
+ *              it.  A captured adjoint skips the clearing pass when the
+ *              buffer is zero at capture (after one eager adjoint of the
+ *              same nreps), and each replay leaves it zero again.  A call
+ *              with a larger nreps reallocates the buffer, which ends the
+ *              validity of adjoints captured before it.
  */
 int mi355nufft_adjoint(mi355nufft_plan *plan, const void *y, void *x,
                        const int64_t *xs, int32_t nreps, void *grid);
