@@ -10,7 +10,9 @@ is three launches plus rocFFT for all F*C images at once:
   work item, each coil reusing the per-point coefficients.
 
 ``capture()`` records the forward and adjoint into HIP graphs (torch's
-CUDAGraph API on ROCm) so that a transform is a single graph launch.
+CUDAGraph API on ROCm) so that a transform is a single graph launch.  Its
+eager warm-up calls leave the plan's adjoint grid all zero, so the captured
+adjoint has no grid-clearing pass; every replay leaves the grid zero again.
 The reference has no batched path: its repetitions are a Python loop of
 one kernel launch per repetition (mrrt/nufft/_nufft.py:1069-1084, :1181-1189).
 """
