@@ -18,7 +18,7 @@
 // A workgroup transforms lpw lines in LDS, TL threads per line, with a
 // Stockham auto-sort FFT (radix-8 passes, a final radix-2 or radix-4 pass);
 // twiddles come from a table of exp(-2 pi i m / K) rounded once from double,
-// staged in LDS.
+// staged in LDS as one table per pass, ordered by butterfly.
 #pragma once
 
 #include "device_common.hpp"
@@ -110,6 +110,17 @@ __device__ __forceinline__ void dft_r(cx<T>* v) {
     else dft2<T, SIGN>(v);
 }
 
+// Radix of the pass of sub-transform size ns (8, the last one 2 or 4) and
+// where its twiddle table starts (the passes with ns > 1, in order).
+template <int K>
+__host__ __device__ constexpr int tw_radix(int ns) { return K / ns >= 8 ? 8 : K / ns; }
+template <int K>
+__host__ __device__ constexpr int tw_offset(int ns) {
+    int off = 0;
+    for (int s = 8; s < ns; s *= tw_radix<K>(s)) off += (tw_radix<K>(s) - 1) * s;
+    return off;
+}
+
 // One Stockham pass of radix R over a line of K points in LDS, sub-transform
 // size NS before the pass: butterfly j reads j + r K/R, multiplies by
 // w^(k r K/(NS R)) with k = j mod NS, and writes (j / NS) NS R + k + r NS.
@@ -136,7 +147,7 @@ __device__ __forceinline__ void stockham_pass(cx<T>* line, int tl, const cx<T>* 
             if constexpr (NS > 1) {
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    cx<T> w = tw[k * r * (K / (NS * R))];
+                    cx<T> w = tw[tw_offset<K>(NS) + (r - 1) * NS + k];
                     if (SIGN > 0) w.y = -w.y;
                     v[q][r] = cmulc<T>(v[q][r], w);
                 }
@@ -178,10 +189,25 @@ struct FftRowGeom {
                        // is the contiguous one, as for C-ordered images), else along each line
 };
 
+// The LDS twiddles, one table per pass of sub-transform size ns > 1 and
+// radix R: entry tw_offset(ns) + (r - 1) ns + k = w^(k r K / (ns R)), so that
+// the lanes of a pass (consecutive k) read consecutive entries.  The tables
+// take fewer than K entries in all.
 template <typename T, int K>
 __device__ __forceinline__ cx<T>* stage_twiddles(unsigned char* smem, int lpw, const cx<T>* __restrict__ tw) {
     cx<T>* tws = reinterpret_cast<cx<T>*>(smem) + lpw * FftCfg<T, K>::KP;
-    for (int i = threadIdx.x; i < K; i += blockDim.x) tws[i] = tw[i];
+    for (int i = threadIdx.x; i < K; i += blockDim.x) {
+        int ns = 8, off = 0, R = 1;
+        while (ns < K) {
+            R = tw_radix<K>(ns);
+            if (i - off < (R - 1) * ns) break;
+            off += (R - 1) * ns;
+            ns *= R;
+        }
+        if (ns >= K) continue;  // past the last table
+        const int e = i - off, r = 1 + e / ns, k = e % ns;
+        tws[i] = tw[k * r * (K / (ns * R))];
+    }
     return tws;
 }
 
