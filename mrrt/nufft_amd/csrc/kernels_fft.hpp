@@ -37,22 +37,15 @@ constexpr int kFftMaxThreads = 1024;
 template <typename T, int K>
 struct FftCfg {
     static constexpr int CXS = 2 * (int)sizeof(T);
-    // a line holds K points at pad(t) (a permutation within 32-point
-    // blocks); the line stride K + K / 32 + 1 starts neighbouring lines on
+    // a line holds K points at pad(t) = t + t / 32 (strided butterfly writes
+    // spread over the banks), plus one so neighbouring lines start on
     // different banks
     static constexpr int KP = K + K / 32 + 1;
     static constexpr int TL = K / 8 < 64 ? K / 8 : 64;
     static constexpr size_t lds_bytes(int lpw) { return (size_t)(lpw * KP + K) * CXS; }
 };
 
-// Position of point t in its LDS line: an XOR swizzle inside each 32-point
-// block.  With t = 64 B + 8 S + U, slot bits 3-4 take (B ^ S) & 3 and bits
-// 0-2 take U ^ S, so the three lane patterns of the radix-8 passes (lanes
-// varying S and U, B and S, or B and U) land on distinct bank pairs.
-__device__ __forceinline__ int pad(int t) {
-    const int B = (t >> 6) & 3, S = (t >> 3) & 7, U = t & 7;
-    return (t & ~31) | ((((B ^ S) & 3) << 3) | (U ^ S));
-}
+__device__ __forceinline__ int pad(int t) { return t + (t >> 5); }
 
 // Small DFTs: X[k] = sum_n x[n] exp(SIGN * 2 pi i n k / R)
 template <typename T, int SIGN>
