@@ -27,7 +27,7 @@ __all__ = [
 
 
 def beatty_alpha(J, K, N):
-    """KB shape parameter of Beatty et al. 2005, eq. 5 (_kernels.py:288-291)."""
+    """KB shape parameter of Beatty et al. 2005, eq. 5 (_kernels.py:151-154; m = 0 at :160)."""
     k_n = K / N
     return np.pi * sqrt(J ** 2 / k_n ** 2 * (k_n - 0.5) ** 2 - 0.8)
 

@@ -16,6 +16,7 @@ __all__ = [
     "golden_angle_frames",
     "ellipse_phantom",
     "complex_gaussian",
+    "coil_maps",
 ]
 
 GOLDEN = np.pi * (np.sqrt(5.0) - 1.0) / 2.0
@@ -107,3 +108,26 @@ def ellipse_phantom(shape, nellipses=10, seed=0):
         yr = -(x - cx_) * np.sin(th) + (y - cy_) * np.cos(th)
         img[(xr / a) ** 2 + (yr / b) ** 2 <= 1] += amp
     return img
+
+
+def coil_maps(ncoils, shape, seed=0):
+    """Smooth random complex coil sensitivities, (ncoils,) + shape complex128.
+
+    Coil c is a Gaussian bump exp(-|r - r_c|^2 / w_c^2) times a linear phase
+    exp(i (phi_c + g_c . r)) on the grid r in [-1, 1]^d, with the centre r_c,
+    width w_c, phase offset phi_c and gradient g_c drawn from a seeded rng;
+    the maps are normalised so that sum_c |S_c|^2 has maximum 1.
+    """
+    rng = np.random.default_rng(seed)
+    shape = tuple(int(n) for n in shape)
+    r = np.meshgrid(*[np.linspace(-1, 1, n) for n in shape], indexing="ij")
+    maps = np.empty((ncoils,) + shape, np.complex128)
+    for c in range(ncoils):
+        cen = rng.uniform(-1.2, 1.2, len(shape))
+        width = rng.uniform(0.6, 1.4)
+        phi = rng.uniform(0, 2 * np.pi)
+        grad = rng.uniform(-2, 2, len(shape))
+        d2 = sum((r[d] - cen[d]) ** 2 for d in range(len(shape)))
+        ph = phi + sum(grad[d] * r[d] for d in range(len(shape)))
+        maps[c] = np.exp(-d2 / width ** 2) * np.exp(1j * ph)
+    return maps / np.sqrt(np.max(np.sum(np.abs(maps) ** 2, axis=0)))

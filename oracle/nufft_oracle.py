@@ -73,7 +73,7 @@ __all__ = [
 # kernel math (setup time)
 # --------------------------------------------------------------------------
 def beatty_alpha(J, K, N):
-    """Beatty KB shape parameter, ``_kernels.py:288-291`` (m = 0, ``:298``)."""
+    """Beatty KB shape parameter, ``_kernels.py:151-154`` (m = 0, ``:160``)."""
     k_n = K / N
     return np.pi * sqrt(J ** 2 / k_n ** 2 * (k_n - 0.5) ** 2 - 0.8)
 
@@ -402,15 +402,32 @@ def interp_adjoint(fm, Kd, hs, Jd, L, tm, cplx_kernel=False):
         y = fm[:, r]
         acc_re = np.zeros(nk, np.float64)
         acc_im = np.zeros(nk, np.float64)
-        # enumerate all taps; bincount accumulates in float64 which is
-        # within the tolerance of the serial float32 C accumulation
+        # Every tap value conj(c) * y is formed in the operator's precision as
+        # in the C kernels; the sums accumulate in float64 (bincount), which
+        # is within the tolerance of the serial float32 C accumulation.  The
+        # taps of one outer-axis index are collected and added by one
+        # bincount over the grid.
+        idx_l, val_l = [], []
+
         def add(idx, v):
+            idx_l.append(idx)
+            val_l.append(v)
+
+        def flush():
+            if not idx_l:
+                return
+            idx = np.concatenate(idx_l)
+            v = np.concatenate(val_l)
             acc_re[:] += np.bincount(idx, weights=v.real.astype(np.float64), minlength=nk)
             acc_im[:] += np.bincount(idx, weights=v.imag.astype(np.float64), minlength=nk)
+            idx_l.clear()
+            val_l.clear()
+
         if ndim == 1:
             (k1, c1), = taps
             for j1 in range(Jd[0]):
                 add(k1[:, j1], np.conj(c1[:, j1]) * y)
+            flush()
         elif ndim == 2:
             (k1, c1), (k2, c2) = taps
             for j2 in range(Jd[1]):
@@ -418,6 +435,7 @@ def interp_adjoint(fm, Kd, hs, Jd, L, tm, cplx_kernel=False):
                 base = k2[:, j2] * strides[1]
                 for j1 in range(Jd[0]):
                     add(base + k1[:, j1], np.conj(c1[:, j1]) * v2)
+            flush()
         else:
             (k1, c1), (k2, c2), (k3, c3) = taps
             for j3 in range(Jd[2]):
@@ -427,6 +445,7 @@ def interp_adjoint(fm, Kd, hs, Jd, L, tm, cplx_kernel=False):
                     base = k3[:, j3] * strides[2] + k2[:, j2] * strides[1]
                     for j1 in range(Jd[0]):
                         add(base + k1[:, j1], np.conj(c1[:, j1]) * v2)
+                flush()
         out[:, r] = (acc_re.astype(real_t) + 1j * acc_im.astype(real_t))
     return out
 
@@ -461,17 +480,20 @@ def dtft(x, omega, shape, n_shift=None, chunk=4096):
     return out[:, 0] if out.shape[1] == 1 else out
 
 
-def dtft_adj(X, omega, shape, n_shift=None, chunk=4096):
+def dtft_adj(X, omega, shape, n_shift=None, chunk=4096, pixels=None):
     """Exact adjoint ``x[n] = sum_m X[m] exp(+i omega_m . (n - n_shift))``.
 
     ``_dtft.py:126-216`` (dense form ``:207-210``), chunked over m.
-    Returns (prod(shape),) or (prod(shape), L) in Fortran order.
+    Returns (prod(shape),) or (prod(shape), L) in Fortran order; with
+    ``pixels`` (Fortran flat indices) only those entries, in that order.
     """
     omega = np.asarray(omega, dtype=np.float64)
     dd = omega.shape[1]
     n_shift = np.zeros(dd) if n_shift is None else np.atleast_1d(np.squeeze(n_shift))
     X = np.asarray(X).reshape((omega.shape[0], -1)).astype(np.complex128)
     nn = _grid_coords(shape, n_shift)
+    if pixels is not None:
+        nn = [g[np.asarray(pixels)] for g in nn]
     out = np.zeros((len(nn[0]), X.shape[1]), np.complex128)
     for s in range(0, omega.shape[0], chunk):
         om = omega[s:s + chunk]

@@ -186,6 +186,10 @@ def test_dtft_adj_vs_ifft(Nd, n_shift):
     Xp = X * np.exp(-1j * (om @ np.asarray(n_shift, float))).reshape(Nd, order="F")
     xf = np.fft.ifftn(Xp) * np.prod(Nd)
     np.testing.assert_allclose(xd, xf, atol=1e-7)
+    # the pixel-subset form picks the same entries (Fortran flat indices)
+    pix = np.random.default_rng(3).choice(int(np.prod(Nd)), min(7, int(np.prod(Nd))), replace=False)
+    xs = O.dtft_adj(X.ravel(order="F"), om, Nd, n_shift, pixels=pix)
+    np.testing.assert_allclose(xs, xf.ravel(order="F")[pix], atol=1e-7)
 
 
 # ---------------------------------------------------------- composed operator
