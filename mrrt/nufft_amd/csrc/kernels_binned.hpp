@@ -94,7 +94,7 @@ __global__ __launch_bounds__(256) void bin_count_kernel(const T* __restrict__ tm
 }
 
 // counts[b] = points of bin b (sum over its bank keys)
-__global__ __launch_bounds__(256) void bin_sum_kernel(BinState bs, int nbins) {
+static __global__ __launch_bounds__(256) void bin_sum_kernel(BinState bs, int nbins) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbins) return;
     const int4* q = reinterpret_cast<const int4*>(bs.bkt + (int64_t)b * kBanks);
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(256) void bin_sum_kernel(BinState bs, int nbins) {
 
 // One workgroup of 1024 threads: exclusive scans of the bin counts and of the
 // per-bin subproblem counts, then the subproblem table.
-__global__ __launch_bounds__(1024) void bin_scan_kernel(BinState bs, int nbins, int P, int max_sub) {
+static __global__ __launch_bounds__(1024) void bin_scan_kernel(BinState bs, int nbins, int P, int max_sub) {
     __shared__ int s_pts[1024];
     __shared__ int s_sub[1024];
     const int tid = threadIdx.x;
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(1024) void bin_scan_kernel(BinState bs, int nbins, 
 // subproblem j holds positions j, j + ns, j + 2 ns, ... (q or q + 1 points,
 // c = q ns + rem), so every subproblem samples all rank levels and the dense
 // cells of a bin do not pile up in its last subproblem.
-__global__ __launch_bounds__(256) void bin_scatter_kernel(int64_t M, BinState bs, int P) {
+static __global__ __launch_bounds__(256) void bin_scatter_kernel(int64_t M, BinState bs, int P) {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= M) return;
     const int slot = bs.bin_of[m];
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(256) void bin_scatter_kernel(int64_t M, BinState bs
     bs.perm[bs.bin_start[bin] + j * q + min(j, rem) + pos / ns] = (int)m;
 }
 
-__global__ __launch_bounds__(256) void fill_int_kernel(int* __restrict__ p, int64_t n, int v) {
+static __global__ __launch_bounds__(256) void fill_int_kernel(int* __restrict__ p, int64_t n, int v) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }

@@ -115,7 +115,7 @@ __global__ __launch_bounds__(256) void sp_fill_rows_kernel(
     }
 }
 
-__global__ __launch_bounds__(256) void sp_iota_kernel(int32_t* __restrict__ v, int64_t n) {
+static __global__ __launch_bounds__(256) void sp_iota_kernel(int32_t* __restrict__ v, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) v[i] = (int32_t)i;
 }
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(256) void sp_fill_cols_kernel(const uint32_t* __res
 }
 
 // ptr[k] = first sorted entry with grid index >= k, for k = 0..nk.
-__global__ __launch_bounds__(256) void sp_col_ptr_kernel(const uint32_t* __restrict__ keys, int64_t nnz, int64_t nk,
+static __global__ __launch_bounds__(256) void sp_col_ptr_kernel(const uint32_t* __restrict__ keys, int64_t nnz, int64_t nk,
                                                          int32_t* __restrict__ ptr) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k > nk) return;
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(256) void sp_col_ptr_kernel(const uint32_t* __restr
 // Per column: pieces, partial-sum slots (split columns only) and split flag.
 // Entry nk of each count array is 0 so that the exclusive scans end in the
 // totals.
-__global__ __launch_bounds__(256) void sp_piece_count_kernel(const int32_t* __restrict__ ptr, int64_t nk,
+static __global__ __launch_bounds__(256) void sp_piece_count_kernel(const int32_t* __restrict__ ptr, int64_t nk,
                                                              int32_t* __restrict__ npc, int32_t* __restrict__ nsl,
                                                              int32_t* __restrict__ nsp) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(256) void sp_piece_count_kernel(const int32_t* __re
     nsp[k] = split ? 1 : 0;
 }
 
-__global__ __launch_bounds__(256) void sp_piece_emit_kernel(const int32_t* __restrict__ ptr, int64_t nk,
+static __global__ __launch_bounds__(256) void sp_piece_emit_kernel(const int32_t* __restrict__ ptr, int64_t nk,
                                                             const int32_t* __restrict__ opc,
                                                             const int32_t* __restrict__ osl,
                                                             const int32_t* __restrict__ osp,

@@ -3,29 +3,12 @@
 // deapodization factors, the per-sample state (tm, phase), the rocFFT plans
 // with their work buffer and an optional grid scratch buffer.  Everything is
 // enqueued on the plan's stream.
-#include <hip/hip_ext.h>
-#include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
-#include <rocfft/rocfft.h>
 
-#include <cmath>
-#include <cstdarg>
-#include <cstdio>
-#include <cstring>
-#include <string>
-#include <vector>
-
-#include "../../../include/mi355nufft.h"
-#include "kernels_grid.hpp"
-#include "kernels_interp.hpp"
-#include "kernels_binned.hpp"
-#include "kernels_fft.hpp"
-#include "kernels_sparse.hpp"
+#include "internal.hpp"
 #include "kernels_coil.hpp"
 
-using namespace mi355;
-
-namespace {
+namespace mi355host {
 
 thread_local std::string g_err;
 
@@ -39,138 +22,46 @@ int fail(int code, const char* fmt, ...) {
     return code;
 }
 
-#define HIP_TRY(expr)                                                                   \
-    do {                                                                                \
-        hipError_t e_ = (expr);                                                         \
-        if (e_ != hipSuccess)                                                           \
-            return fail(MI355NUFFT_EHIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
-                        __FILE__, __LINE__);                                            \
-    } while (0)
+// Kernel timing: while timing is on and the stream is not being captured,
+// each tiled gather or scatter launch goes through hipExtLaunchKernel with a
+// start/stop event pair that the dispatch itself timestamps (the kernel's own
+// execution, not the gap before it).
+constexpr int kTimingPairs = 1024;
 
-#define LAUNCH_CHECK()                                                                     \
-    do {                                                                                   \
-        hipError_t e_ = hipGetLastError();                                                 \
-        if (e_ != hipSuccess)                                                              \
-            return fail(MI355NUFFT_EHIP, "kernel launch: %s (%s:%d)", hipGetErrorString(e_), \
-                        __FILE__, __LINE__);                                               \
-    } while (0)
+int timing_fold(mi355nufft_plan* p, int w) {
+    for (int i = 0; i < p->tev_used[w]; ++i) {
+        float ms = 0.f;
+        HIP_TRY(hipEventSynchronize(p->tev[w][2 * i + 1]));
+        HIP_TRY(hipEventElapsedTime(&ms, p->tev[w][2 * i], p->tev[w][2 * i + 1]));
+        p->tev_ms[w] += ms;
+        ++p->tev_n[w];
+    }
+    p->tev_used[w] = 0;
+    return MI355NUFFT_OK;
+}
 
-constexpr int kBlock = 256;
+// events for the next launch of kind w, or nulls when not timing
+int timing_events(mi355nufft_plan* p, int w, hipEvent_t* start, hipEvent_t* stop) {
+    *start = *stop = nullptr;
+    if (!p->timing) return MI355NUFFT_OK;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(p->stream, &cs));
+    if (cs != hipStreamCaptureStatusNone) return MI355NUFFT_OK;
+    if (p->tev_used[w] == kTimingPairs) RET_IF(timing_fold(p, w));
+    const size_t need = 2 * (size_t)(p->tev_used[w] + 1);
+    while (p->tev[w].size() < need) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        p->tev[w].push_back(e);
+    }
+    const int i = p->tev_used[w]++;
+    *start = p->tev[w][2 * i];
+    *stop = p->tev[w][2 * i + 1];
+    return MI355NUFFT_OK;
+}
 
-inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+}  // namespace mi355host
 
-struct FftPlans {
-    int nreps;
-    rocfft_plan fwd;
-    rocfft_plan bwd;
-};
-
-// Sparse-mode matrix (kernels_sparse.hpp): row form for the forward, column
-// pieces for the adjoint.
-struct SparseState {
-    int on = 0;
-    int cvals = 0;  // complex values (complex kernel)
-    int nnzr = 0;   // entries per row, prod J
-    int64_t M = 0, nnz = 0, npieces = 0, nsplit = 0, nslots = 0;
-    void* val = nullptr;       // (nnzr, M) real or complex, sorted rows
-    int32_t* rbase = nullptr;  // (ndim, M) first grid index per axis of each sorted row
-    int32_t* perm = nullptr;   // sorted row -> sample
-    void* tval = nullptr;      // (nnz) values in column order
-    int32_t* trow = nullptr;   // (nnz) sample of each column entry
-    SpPiece* pieces = nullptr;
-    SpSplit* splits = nullptr;
-    void* part = nullptr;      // partial sums of split columns (nslots x nreps)
-    size_t part_bytes = 0;
-    void* ybuf = nullptr;      // samples times conj(phase_after) (M x nreps)
-    size_t ybuf_bytes = 0;
-};
-
-}  // namespace
-
-struct mi355nufft_plan {
-    int ndim = 0;
-    int64_t N[3] = {1, 1, 1};
-    int64_t K[3] = {1, 1, 1};
-    int J[3] = {1, 1, 1};
-    int L = 0;
-    int precision = 0;  // 0 float, 1 double
-    int phasing = 0;    // 0 real kernel, 1 complex kernel
-    int ortho = 0;
-    double adjoint_scale = 1.0;
-    double scale_ortho = 1.0;
-    double phase_coef[3] = {0, 0, 0};
-    int64_t nk = 1;  // prod K
-    int64_t nn = 1;  // prod N
-    hipStream_t stream = nullptr;
-
-    void* d_tab[3] = {nullptr, nullptr, nullptr};  // full tables (J*L+1 entries)
-    void* d_tw[3] = {nullptr, nullptr, nullptr};   // FFT twiddles exp(-2 pi i m / K_d), complex T
-    int fft_fused_ok = 0;  // every K_d a power of two in [kFftMinK, kFftMaxK]
-    int fft_method = 0;    // 0 auto (fused pruned FFT when possible), 1 rocFFT
-    // set while mi355nufft_forward / _adjoint run with phase_before (and the
-    // forward's ortho scale) applied by the fused FFT passes instead of the
-    // tiled gather's staging / scatter's flush
-    int fft_phased = 0;
-    double* d_sn[3] = {nullptr, nullptr, nullptr};
-
-    int64_t M = 0;
-    int64_t M_cap = 0;
-    void* d_tm = nullptr;     // (ndim, M) real
-    void* d_phase = nullptr;  // (M) complex or null
-    int has_phase = 0;
-    double shift[3] = {0, 0, 0};  // exponent coefficients of the per-sample phase
-
-    void* d_grid = nullptr;  // plan-owned grid scratch (forward)
-    size_t grid_bytes = 0;
-    // plan-owned adjoint grid; the fused adjoint FFT passes write zeros over
-    // everything they read, so after an adjoint through it the buffer is all
-    // zero again and the next scatter needs no separate clear
-    void* d_agrid = nullptr;
-    size_t agrid_bytes = 0;
-    int agrid_zero = 0;  // d_agrid holds zeros (host-side state, stream order)
-
-    // binning (tiled kernels)
-    int method = 0;           // 0 auto, 1 global-memory, 2 tiled, 3 tiled + atomic flush, 4 tiled + two-phase
-    int reduce_ok = 0;        // two-phase scatter possible for the current geometry
-    int reduce_j[3] = {1, 1, 1};  // candidate bins per axis of the reduce kernel
-    void* d_tiles = nullptr;  // two-phase scatter tile slots
-    size_t tiles_bytes = 0;
-    void* tiles_target = nullptr;  // set while a two-phase scatter runs
-    int bin_req[3] = {0, 0, 0};  // requested bin size per axis (0: default)
-    int P_req = 0;               // requested points per subproblem (0: default)
-    int threads_req = 0;         // requested threads per tiled workgroup (0: default)
-    BinGeom bg{};
-    int bg_threads = 256;     // threads per tiled workgroup
-    int binned = 0;           // bin state valid for the current points
-    int* d_bin_i = nullptr;   // one allocation: bin_of, rank, perm (3*M_cap)
-    int* d_bin_b = nullptr;   // counts, bin_start, bin_sub, bank-key counts ((3 + kBanks) * nbins_cap)
-    int* d_sub = nullptr;     // sub_bin, sub_start, sub_end (3*sub_cap) + n_sub
-    void* d_tms = nullptr;    // tm in sorted order (ndim * M_cap real)
-    void* d_phs = nullptr;    // per-sample phase in sorted order (M_cap complex)
-    int64_t sorted_cap = 0;
-    int64_t bin_m_cap = 0, bin_b_cap = 0, bin_s_cap = 0;
-    size_t lds_limit = 65536;
-    int nframes = 1;              // frames with their own points (grids per frame: nreps)
-    int64_t* d_frame_off = nullptr;
-    int64_t frame_cap = 0;
-
-    // launch timing of the tiled gather (0) and scatter (1) kernels
-    int timing = 0;
-    std::vector<hipEvent_t> tev[2];  // start/stop pairs, reused
-    int tev_used[2] = {0, 0};
-    double tev_ms[2] = {0, 0};       // folded totals of earlier pairs
-    int64_t tev_n[2] = {0, 0};
-
-    SparseState sp;
-
-    std::vector<FftPlans> ffts;
-    rocfft_execution_info fft_info = nullptr;
-    void* fft_work = nullptr;
-    size_t fft_work_bytes = 0;
-
-    size_t real_size() const { return precision ? sizeof(double) : sizeof(float); }
-    size_t cx_size() const { return 2 * real_size(); }
-};
 
 namespace {
 
@@ -191,39 +82,6 @@ int check_geometry(int ndim, const int64_t* Kd, const int32_t* Jd, int32_t L) {
     return MI355NUFFT_OK;
 }
 
-template <typename T, bool CK>
-void fill_interp_args(InterpArgs<T, CK>& a, int ndim, const int64_t* Kd, const int32_t* Jd, int L,
-                      const void* const* h, const void* tm, int64_t M, int nreps) {
-    std::memset(&a, 0, sizeof(a));
-    a.M = M;
-    a.L = L;
-    a.nreps = nreps;
-    int64_t nk = 1;
-    for (int d = 0; d < 3; ++d) {
-        if (d < ndim) {
-            const int JL = Jd[d] * L;
-            const int nc = JL / 2;
-            a.ax[d].K = (int)Kd[d];
-            a.ax[d].J = Jd[d];
-            a.ax[d].nlo = -nc;
-            a.ax[d].nhi = JL - 1 - nc;
-            a.h[d] = reinterpret_cast<const tab<T, CK>*>(h[d]) + nc;
-            nk *= Kd[d];
-        } else {
-            a.ax[d].K = 1;
-            a.ax[d].J = 1;
-            a.h[d] = nullptr;
-        }
-    }
-    a.kstride = nk;
-    a.tm = reinterpret_cast<const T*>(tm);
-}
-
-int jmax_of(int ndim, const int32_t* Jd) {
-    int j = 0;
-    for (int d = 0; d < ndim; ++d) j = Jd[d] > j ? Jd[d] : j;
-    return j <= 4 ? 4 : (j <= 8 ? 8 : 16);
-}
 
 template <typename T, int NDIM, bool CK, int JMAX>
 void launch_interp(bool adjoint, const InterpArgs<T, CK>& a, hipStream_t s) {
@@ -309,11 +167,6 @@ int check_nreps(int32_t nreps) {
     return MI355NUFFT_OK;
 }
 
-#define RET_IF(x)                 \
-    do {                          \
-        int rc_ = (x);            \
-        if (rc_) return rc_;      \
-    } while (0)
 
 int get_agrid(mi355nufft_plan* p, int32_t nimg) {
     const size_t need = (size_t)nimg * (size_t)p->nk * p->cx_size();
@@ -482,182 +335,6 @@ int fft_impl(mi355nufft_plan* p, void* grid, int32_t nreps, int direction);
 // Fused pruned FFT passes (kernels_fft.hpp) for power-of-two grids.
 bool use_fused_fft(const mi355nufft_plan* p) { return p->fft_method == 0 && p->fft_fused_ok; }
 
-// What the fused FFT passes need of a plan: sizes, twiddles, deapodization
-// factors (null: none) and the stream.
-struct FftDev {
-    int ndim;
-    int64_t N[3], K[3], nk;
-    void* const* tw;
-    const double* const* sn;
-    hipStream_t stream;
-};
-
-FftDev fft_dev(const mi355nufft_plan* p) {
-    FftDev f;
-    f.ndim = p->ndim;
-    for (int d = 0; d < 3; ++d) {
-        f.N[d] = p->N[d];
-        f.K[d] = p->K[d];
-    }
-    f.nk = p->nk;
-    f.tw = p->d_tw;
-    f.sn = p->d_sn;
-    f.stream = p->stream;
-    return f;
-}
-
-FftRowGeom row_geom(const FftDev& f, const int64_t* xs, int32_t nimg) {
-    FftRowGeom g;
-    for (int d = 0; d < 3; ++d) {
-        g.N[d] = (int)f.N[d];
-        g.K[d] = (int)f.K[d];
-    }
-    g.nk = f.nk;
-    for (int d = 0; d < 4; ++d) g.xs[d] = 0;
-    for (int d = 0; d < f.ndim; ++d) g.xs[d] = xs[d];
-    g.xs[3] = xs[f.ndim];
-    g.nlines = (int64_t)g.N[1] * g.N[2] * nimg;
-    g.mult = nullptr;
-    // C-ordered images (axis 1 contiguous): neighbouring lines are adjacent
-    // in memory, so the image side is read / written across lines
-    g.across = (f.ndim >= 2 && g.xs[1] == 1 && g.xs[0] != 1) ? 1 : 0;
-    return g;
-}
-
-// lines of axis d >= 1: LPW neighbouring lines of prod(K_0..K_{d-1}) per
-// workgroup, for every active index of the next axis and every image
-FftColGeom col_geom(const FftDev& f, int d, bool adjoint, int32_t nimg) {
-    FftColGeom g;
-    g.S = 1;
-    for (int e = 0; e < d; ++e) g.S *= f.K[e];
-    g.n_in = (int)(adjoint ? f.K[d] : f.N[d]);
-    g.n_out = (int)(adjoint ? f.N[d] : f.K[d]);
-    g.A = d + 1 < f.ndim ? (int)f.N[d + 1] : 1;
-    g.st_a = g.S * f.K[d];
-    g.nk = f.nk;
-    g.nouter = (int64_t)g.A * nimg;
-    g.mult = nullptr;
-    g.pb = 0;
-    g.ortho = 0;
-    g.pb_ndim = f.ndim;
-    g.k0_mask = (int)f.K[0] - 1;
-    g.k0_shift = 0;
-    while (((int64_t)1 << g.k0_shift) < f.K[0]) ++g.k0_shift;
-    g.pbc[0] = g.pbc[1] = g.pbc[2] = 0.0;
-    g.scale_ortho = 1.0;
-    return g;
-}
-
-// Workgroups a pass aims for (measured best on 512^2 grids): lines per workgroup grow only
-// while the pass keeps at least this many workgroups.
-constexpr int64_t kFftTargetBlocks = 128;
-// LDS cap of the row passes over C-ordered images (room for 16 lines of 512
-// complex64 points)
-constexpr size_t kFftLdsAcross = 81920;
-
-// Options of one pass: rezero (adjoint: leave the grid it reads all zero),
-// mult (adjoint: multiply the loaded spectrum by this grid-shaped array),
-// pb (column pass of an ndim >= 2 power-of-two grid: 1 multiplies the
-// output by phase_before and divides by scale_ortho if ortho, 2 multiplies
-// the input by conj(phase_before); pbc null means a zero phase).
-struct FftPassOpts {
-    int rezero = 0;
-    const void* mult = nullptr;
-    int pb = 0;
-    const double* pbc = nullptr;
-    int ortho = 0;
-    double scale_ortho = 1.0;
-};
-
-template <typename T, int K>
-int launch_fft_rows(const FftDev& f, bool adjoint, const void* x, void* xo, const int64_t* xs, void* grid,
-                    int32_t nimg, T scale, FftPassOpts o) {
-    using C = FftCfg<T, K>;
-    FftRowGeom g = row_geom(f, xs, nimg);
-    g.mult = o.mult;
-    // more lines per workgroup across C-ordered images, for longer contiguous
-    // runs of the image side (16 lines of complex64: one 128-byte line)
-    const int max_threads = g.across ? 1024 : 256;
-    const size_t lds_cap = g.across ? kFftLdsAcross : (size_t)kFftLds;
-    int lpw = 1;
-    while (2 * lpw * C::TL <= max_threads && C::lds_bytes(2 * lpw) <= lds_cap &&
-           g.nlines / (2 * lpw) >= kFftTargetBlocks)
-        lpw *= 2;
-    g.lpw = lpw;
-    const int64_t nb = (g.nlines + lpw - 1) / lpw;
-    const size_t lds = C::lds_bytes(lpw);
-    const double* s0 = f.sn ? f.sn[0] : nullptr;
-    const double* s1 = f.sn && f.ndim > 1 ? f.sn[1] : s0;
-    const double* s2 = f.sn && f.ndim > 2 ? f.sn[2] : s0;
-    if (adjoint)
-        hipLaunchKernelGGL((fft_rows_crop_kernel<T, K, 1>), dim3((unsigned)nb), dim3(lpw * C::TL), lds, f.stream,
-                           (cx<T>*)grid, (cx<T>*)xo, g, s0, s1, s2, f.ndim, scale, (const cx<T>*)f.tw[0],
-                           o.rezero);
-    else
-        hipLaunchKernelGGL((fft_rows_pad_kernel<T, K, -1>), dim3((unsigned)nb), dim3(lpw * C::TL), lds, f.stream,
-                           (const cx<T>*)x, (cx<T>*)grid, g, s0, s1, s2, f.ndim, (const cx<T>*)f.tw[0]);
-    LAUNCH_CHECK();
-    return MI355NUFFT_OK;
-}
-
-template <typename T, int K>
-int launch_fft_cols(const FftDev& f, bool adjoint, int d, void* grid, int32_t nimg, FftPassOpts o) {
-    using C = FftCfg<T, K>;
-    FftColGeom g = col_geom(f, d, adjoint, nimg);
-    g.zero_tail = o.rezero;
-    g.mult = o.mult;
-    if (o.pb) {
-        g.pb = o.pb;
-        for (int e = 0; e < 3; ++e) g.pbc[e] = o.pbc ? o.pbc[e] : 0.0;
-        g.ortho = o.ortho;
-        g.scale_ortho = o.scale_ortho;
-    }
-    // up to 128 contiguous bytes per row, while the pass keeps its workgroups
-    int lpw = 1;
-    while (2 * lpw * C::CXS <= 128 && g.S % (2 * lpw) == 0 && 2 * lpw * C::TL <= kFftMaxThreads &&
-           C::lds_bytes(2 * lpw) <= (size_t)kFftLds && (g.S / (2 * lpw)) * g.nouter >= kFftTargetBlocks)
-        lpw *= 2;
-    g.lpw = lpw;
-    const int64_t nb = (g.S / lpw) * g.nouter;
-    if (nb >= ((int64_t)1 << 32)) return fail(MI355NUFFT_EINVAL, "FFT launch too large");
-    const size_t lds = C::lds_bytes(lpw);
-    if (adjoint)
-        hipLaunchKernelGGL((fft_cols_kernel<T, K, 1>), dim3((unsigned)nb), dim3(lpw * C::TL), lds, f.stream,
-                           (cx<T>*)grid, g, (const cx<T>*)f.tw[d]);
-    else
-        hipLaunchKernelGGL((fft_cols_kernel<T, K, -1>), dim3((unsigned)nb), dim3(lpw * C::TL), lds, f.stream,
-                           (cx<T>*)grid, g, (const cx<T>*)f.tw[d]);
-    LAUNCH_CHECK();
-    return MI355NUFFT_OK;
-}
-
-#define MI355_FFT_SWITCH(KV, CALL)                                        \
-    switch (KV) {                                                         \
-        case 16: return CALL(16);                                         \
-        case 32: return CALL(32);                                         \
-        case 64: return CALL(64);                                         \
-        case 128: return CALL(128);                                       \
-        case 256: return CALL(256);                                       \
-        case 512: return CALL(512);                                       \
-        case 1024: return CALL(1024);                                     \
-        case 2048: return CALL(2048);                                     \
-        default: return fail(MI355NUFFT_EINVAL, "no fused FFT for K=%d", (int)(KV)); \
-    }
-
-template <typename T>
-int fft_rows(const FftDev& f, bool adjoint, const void* x, void* xo, const int64_t* xs, void* grid, int32_t nimg,
-             T scale, FftPassOpts o = FftPassOpts()) {
-#define MI355_ROWS(KK) launch_fft_rows<T, KK>(f, adjoint, x, xo, xs, grid, nimg, scale, o)
-    MI355_FFT_SWITCH((int)f.K[0], MI355_ROWS)
-#undef MI355_ROWS
-}
-
-template <typename T>
-int fft_cols(const FftDev& f, bool adjoint, int d, void* grid, int32_t nimg, FftPassOpts o = FftPassOpts()) {
-#define MI355_COLS(KK) launch_fft_cols<T, KK>(f, adjoint, d, grid, nimg, o)
-    MI355_FFT_SWITCH((int)f.K[d], MI355_COLS)
-#undef MI355_COLS
-}
 
 // Every K_d a power of two the fused passes support at this precision: one
 // padded line and the twiddle table must fit the LDS budget (complex128
@@ -969,11 +646,6 @@ int sparse_spread(mi355nufft_plan* p, const void* y, void* grid, int32_t nreps, 
 }
 
 template <typename T>
-int run_tiled_any(mi355nufft_plan* p, bool adjoint, const void* samples_in, void* samples_out, const void* grid_in,
-                  void* grid_out, int32_t nreps);
-bool use_tiled(const mi355nufft_plan* p);
-
-template <typename T>
 int interp_impl(mi355nufft_plan* p, const void* grid, void* y, int32_t nreps) {
     if (p->sp.on) return sparse_interp<T>(p, grid, y, nreps, false);
     // tiled path: phase_before and the ortho scale are applied while staging
@@ -1087,39 +759,6 @@ void setup_bin_geom(mi355nufft_plan* p) {
 // LDS of a tiled launch: the gather stages complex T, the scatter
 // accumulates complex float64 (see spread_tiled_kernel); the larger of the two
 // must fit the budget since both share the binning.
-size_t tile_bytes(const mi355nufft_plan* p) { return (size_t)p->bg.tile * 2 * sizeof(double); }
-size_t gather_tile_bytes(const mi355nufft_plan* p) { return (size_t)p->bg.tile * p->cx_size(); }
-
-bool use_tiled(const mi355nufft_plan* p) { return p->method != 1 && p->binned; }
-
-int64_t max_subproblems(const mi355nufft_plan* p) { return p->bg.nbins + (p->M + p->bg.P - 1) / p->bg.P; }
-
-// Workgroups of a persistent tiled launch: one per subproblem slot, at most
-// 2048 (8 per CU on 256 CUs); workgroups stride over the subproblem list.
-unsigned tiled_grid(const mi355nufft_plan* p) {
-    const int64_t n = max_subproblems(p);
-    const int64_t cap = (int64_t)2048 * 256 / p->bg_threads;
-    return (unsigned)(n < cap ? n : cap);
-}
-
-BinState bin_state(const mi355nufft_plan* p) {
-    BinState bs;
-    const int64_t mc = p->bin_m_cap, bc = p->bin_b_cap, sc = p->bin_s_cap;
-    bs.bin_of = p->d_bin_i;
-    bs.rank = p->d_bin_i + mc;
-    bs.perm = p->d_bin_i + 2 * mc;
-    bs.bkt = p->d_bin_b;  // first: its rows are read as int4
-    bs.counts = p->d_bin_b + kBanks * bc;
-    bs.bin_start = bs.counts + bc;
-    bs.bin_sub = bs.counts + 2 * bc;
-    bs.sub_bin = p->d_sub;
-    bs.sub_start = p->d_sub + sc;
-    bs.sub_end = p->d_sub + 2 * sc;
-    bs.n_sub = p->d_sub + 3 * sc;
-    bs.frame_off = p->nframes > 1 ? p->d_frame_off : nullptr;
-    bs.nframes = p->nframes;
-    return bs;
-}
 
 template <typename Tp>
 int grow(mi355nufft_plan* p, Tp** buf, int64_t& cap, int64_t need, int64_t per) {
@@ -1196,208 +835,6 @@ int bin_points_impl(mi355nufft_plan* p) {
 
 int bin_points(mi355nufft_plan* p) { return p->precision ? bin_points_impl<double>(p) : bin_points_impl<float>(p); }
 
-// Kernel timing: while timing is on and the stream is not being captured,
-// each tiled gather or scatter launch goes through hipExtLaunchKernel with a
-// start/stop event pair that the dispatch itself timestamps (the kernel's own
-// execution, not the gap before it).
-constexpr int kTimingPairs = 1024;
-
-int timing_fold(mi355nufft_plan* p, int w) {
-    for (int i = 0; i < p->tev_used[w]; ++i) {
-        float ms = 0.f;
-        HIP_TRY(hipEventSynchronize(p->tev[w][2 * i + 1]));
-        HIP_TRY(hipEventElapsedTime(&ms, p->tev[w][2 * i], p->tev[w][2 * i + 1]));
-        p->tev_ms[w] += ms;
-        ++p->tev_n[w];
-    }
-    p->tev_used[w] = 0;
-    return MI355NUFFT_OK;
-}
-
-// events for the next launch of kind w, or nulls when not timing
-int timing_events(mi355nufft_plan* p, int w, hipEvent_t* start, hipEvent_t* stop) {
-    *start = *stop = nullptr;
-    if (!p->timing) return MI355NUFFT_OK;
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    HIP_TRY(hipStreamIsCapturing(p->stream, &cs));
-    if (cs != hipStreamCaptureStatusNone) return MI355NUFFT_OK;
-    if (p->tev_used[w] == kTimingPairs) RET_IF(timing_fold(p, w));
-    const size_t need = 2 * (size_t)(p->tev_used[w] + 1);
-    while (p->tev[w].size() < need) {
-        hipEvent_t e;
-        HIP_TRY(hipEventCreate(&e));
-        p->tev[w].push_back(e);
-    }
-    const int i = p->tev_used[w]++;
-    *start = p->tev[w][2 * i];
-    *stop = p->tev[w][2 * i + 1];
-    return MI355NUFFT_OK;
-}
-
-template <typename T, bool CK, int NDIM, int JMAX, bool FIXED>
-void launch_tiled(bool adjoint, const TiledArgs<T, CK>& a, unsigned nsub, unsigned threads, size_t lds,
-                  hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
-    auto spread = spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED>;
-    auto interp = a.rc > 1 ? interp_tiled_kernel<T, NDIM, CK, JMAX, FIXED, true>
-                           : interp_tiled_kernel<T, NDIM, CK, JMAX, FIXED, false>;
-    if (ev0) {
-        if (adjoint)
-            hipExtLaunchKernelGGL(spread, dim3(nsub), dim3(threads), (uint32_t)lds, s, ev0, ev1, 0u, a);
-        else
-            hipExtLaunchKernelGGL(interp, dim3(nsub), dim3(threads), (uint32_t)lds, s, ev0, ev1, 0u, a);
-        return;
-    }
-    if (adjoint)
-        hipLaunchKernelGGL(spread, dim3(nsub), dim3(threads), lds, s, a);
-    else
-        hipLaunchKernelGGL(interp, dim3(nsub), dim3(threads), lds, s, a);
-}
-
-// Repetitions (coils) whose tiles one tiled launch stages together, and the
-// LDS they may take (more per workgroup costs occupancy).
-constexpr int kMaxRepChunk = 8;
-constexpr size_t kRepChunkLds = 65536;
-
-// Taps compiled as fixed trip counts when every axis uses the same J among
-// these; other widths run the generic kernels (J <= JMAX checked per tap).
-int fixed_j(int ndim, const int32_t* Jd) {
-    for (int d = 1; d < ndim; ++d)
-        if (Jd[d] != Jd[0]) return 0;
-    return Jd[0] == 6 ? 6 : 0;
-}
-
-template <typename T, bool CK>
-int run_tiled(mi355nufft_plan* p, bool adjoint, const void* samples_in, void* samples_out, const void* grid_in,
-              void* grid_out, int32_t nreps) {
-    TiledArgs<T, CK> a;
-    std::memset(&a, 0, sizeof(a));
-    int32_t J[3] = {p->J[0], p->J[1], p->J[2]};
-    fill_interp_args<T, CK>(a.ia, p->ndim, p->K, J, p->L, (const void* const*)p->d_tab, p->d_tm, p->M, nreps);
-    a.ia.grid = (const cx<T>*)grid_in;
-    a.ia.grid_out = (cx<T>*)grid_out;
-    a.ia.in = (const cx<T>*)samples_in;
-    a.ia.out = (cx<T>*)samples_out;
-    a.ia.phase = p->has_phase ? (const cx<T>*)p->d_phase : nullptr;
-    a.bg = p->bg;
-    a.bs = bin_state(p);
-    a.use_pb = p->phasing == 0 && !p->fft_phased;
-    for (int d = 0; d < 3; ++d) a.pb[d] = (T)p->phase_coef[d];
-    a.ortho = adjoint ? 0 : (p->ortho && !p->fft_phased);
-    a.scale_ortho = (T)p->scale_ortho;
-    a.tiles = adjoint ? (cx<T>*)p->tiles_target : nullptr;
-    a.tms = (const T*)p->d_tms;
-    a.phs = p->has_phase ? (const cx<T>*)p->d_phs : nullptr;
-    a.own_ok = 1;
-    for (int d = 0; d < p->ndim; ++d)
-        if (p->K[d] % p->bg.B[d] != 0) a.own_ok = 0;
-    const unsigned nsub = tiled_grid(p);
-    // the gather stages up to kMaxRepChunk repetitions' tiles at once
-    // (coefficients are then computed once per point for all of them),
-    // within the LDS budget; the scatter one at a time
-    const size_t one = adjoint ? tile_bytes(p) : gather_tile_bytes(p);
-    const size_t chunk_lds = p->lds_limit < kRepChunkLds ? p->lds_limit : kRepChunkLds;
-    int rc = adjoint ? 1 : (int)(chunk_lds / one);
-    rc = rc < kMaxRepChunk ? rc : kMaxRepChunk;
-    rc = rc < nreps ? rc : nreps;
-    a.rc = rc > 1 ? rc : 1;
-    const size_t lds = one * (size_t)a.rc;
-    const int jm = jmax_of(p->ndim, J);
-    const int jf = fixed_j(p->ndim, J);
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    RET_IF(timing_events(p, adjoint ? 1 : 0, &ev0, &ev1));
-#define MI355_FCASE(ND, JF)                                                    \
-    if (p->ndim == ND && jf == JF) {                                           \
-        launch_tiled<T, CK, ND, JF, true>(adjoint, a, nsub, p->bg_threads, lds, p->stream, ev0, ev1); \
-        LAUNCH_CHECK();                                                        \
-        return MI355NUFFT_OK;                                                  \
-    }
-    MI355_FCASE(1, 6) MI355_FCASE(2, 6) MI355_FCASE(3, 6)
-#undef MI355_FCASE
-#define MI355_TCASE(ND, JM)                                                  \
-    if (p->ndim == ND && jm == JM) {                                         \
-        launch_tiled<T, CK, ND, JM, false>(adjoint, a, nsub, p->bg_threads, lds, p->stream, ev0, ev1); \
-        LAUNCH_CHECK();                                                      \
-        return MI355NUFFT_OK;                                                \
-    }
-    MI355_TCASE(1, 4) MI355_TCASE(1, 8) MI355_TCASE(1, 16)
-    MI355_TCASE(2, 4) MI355_TCASE(2, 8) MI355_TCASE(2, 16)
-    MI355_TCASE(3, 4) MI355_TCASE(3, 8) MI355_TCASE(3, 16)
-#undef MI355_TCASE
-    return fail(MI355NUFFT_EINVAL, "unsupported ndim/J combination");
-}
-
-template <typename T>
-int run_tiled_any(mi355nufft_plan* p, bool adjoint, const void* samples_in, void* samples_out, const void* grid_in,
-                  void* grid_out, int32_t nreps) {
-    return p->phasing ? run_tiled<T, true>(p, adjoint, samples_in, samples_out, grid_in, grid_out, nreps)
-                      : run_tiled<T, false>(p, adjoint, samples_in, samples_out, grid_in, grid_out, nreps);
-}
-
-template <typename T, bool CK, int NDIM, int JMAX, bool FIXED>
-void launch_grad(const TiledArgs<T, CK>& a, const GradArgs<T>& g, unsigned nsub, unsigned threads, size_t lds,
-                 hipStream_t s) {
-    if (a.rc > 1)
-        hipLaunchKernelGGL((grad_tiled_kernel<T, NDIM, CK, JMAX, FIXED, true>), dim3(nsub), dim3(threads), lds, s, a,
-                           g);
-    else
-        hipLaunchKernelGGL((grad_tiled_kernel<T, NDIM, CK, JMAX, FIXED, false>), dim3(nsub), dim3(threads), lds, s,
-                           a, g);
-}
-
-template <typename T, bool CK>
-int run_grad(mi355nufft_plan* p, const void* grid, const void* w, void* grad, int32_t nreps) {
-    TiledArgs<T, CK> a;
-    std::memset(&a, 0, sizeof(a));
-    int32_t J[3] = {p->J[0], p->J[1], p->J[2]};
-    fill_interp_args<T, CK>(a.ia, p->ndim, p->K, J, p->L, (const void* const*)p->d_tab, p->d_tm, p->M, nreps);
-    a.ia.grid = (const cx<T>*)grid;
-    a.ia.phase = p->has_phase ? (const cx<T>*)p->d_phase : nullptr;
-    a.bg = p->bg;
-    a.bs = bin_state(p);
-    a.use_pb = p->phasing == 0;
-    for (int d = 0; d < 3; ++d) a.pb[d] = (T)p->phase_coef[d];
-    a.ortho = p->ortho;
-    a.scale_ortho = (T)p->scale_ortho;
-    a.tms = (const T*)p->d_tms;
-    a.phs = p->has_phase ? (const cx<T>*)p->d_phs : nullptr;
-    GradArgs<T> g;
-    g.w = (const cx<T>*)w;
-    g.grad = (T*)grad;
-    for (int d = 0; d < 3; ++d) {
-        g.dtdw[d] = (T)((double)p->K[d] / (2.0 * M_PI));
-        g.shift[d] = (T)p->shift[d];
-    }
-    const unsigned nsub = tiled_grid(p);
-    {  // repetitions staged per pass, as in the gather
-        const size_t chunk_lds = p->lds_limit < kRepChunkLds ? p->lds_limit : kRepChunkLds;
-        int rc = (int)(chunk_lds / gather_tile_bytes(p));
-        rc = rc < kMaxRepChunk ? rc : kMaxRepChunk;
-        rc = rc < nreps ? rc : nreps;
-        a.rc = rc > 1 ? rc : 1;
-    }
-    const size_t lds = gather_tile_bytes(p) * (size_t)a.rc;
-    const int jm = jmax_of(p->ndim, J);
-    const int jf = fixed_j(p->ndim, J);
-#define MI355_GFCASE(ND, JF)                                           \
-    if (p->ndim == ND && jf == JF) {                                   \
-        launch_grad<T, CK, ND, JF, true>(a, g, nsub, p->bg_threads, lds, p->stream);  \
-        LAUNCH_CHECK();                                                \
-        return MI355NUFFT_OK;                                          \
-    }
-    MI355_GFCASE(1, 6) MI355_GFCASE(2, 6) MI355_GFCASE(3, 6)
-#undef MI355_GFCASE
-#define MI355_GCASE(ND, JM)                                            \
-    if (p->ndim == ND && jm == JM) {                                   \
-        launch_grad<T, CK, ND, JM, false>(a, g, nsub, p->bg_threads, lds, p->stream); \
-        LAUNCH_CHECK();                                                \
-        return MI355NUFFT_OK;                                          \
-    }
-    MI355_GCASE(1, 4) MI355_GCASE(1, 8) MI355_GCASE(1, 16)
-    MI355_GCASE(2, 4) MI355_GCASE(2, 8) MI355_GCASE(2, 16)
-    MI355_GCASE(3, 4) MI355_GCASE(3, 8) MI355_GCASE(3, 16)
-#undef MI355_GCASE
-    return fail(MI355NUFFT_EINVAL, "unsupported ndim/J combination");
-}
 
 // Two-phase scatter (binning method 4, the deterministic mode): the tiled
 // scatter writes each subproblem's tile to a slot, tile_reduce_kernel sums
