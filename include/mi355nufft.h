@@ -356,6 +356,16 @@ int mi355nufft_weight_samples(int precision, const void *y, const void *w,
  *           real T, or interleaved complex T if complex_kernel)
  *   tm      device real (ndim, M): tm[d*M + m]  (the reference's p_tm)
  *   fm      device complex (nreps, M), written
+ * The points run on the tiled kernels through a bin sort of tm that the
+ * library caches per (device, stream) in a workspace it allocates (grown on
+ * demand, kept until the process ends).  Every call compares tm with the
+ * copy the cached sort was made from, on the device, and re-sorts within the
+ * same call when anything differs, also when tm was rewritten in place; the
+ * host reads nothing back, so the call stays asynchronous.  Geometries whose
+ * scatter tile does not fit the LDS, and calls beyond 16 (device, stream)
+ * workspaces, run the global-memory kernels (one thread per sample).
+ * Calls on one stream are ordered; calls on different streams use different
+ * workspaces.
  */
 int mi355nufft_table1_forward(int ndim, const int64_t *Kd, const int32_t *Jd,
                               int32_t L, int precision, int complex_kernel,
@@ -369,6 +379,7 @@ int mi355nufft_table1_forward(int ndim, const int64_t *Kd, const int32_t *Jd,
  * 144-305, 308-531, 924-1202), _interp{1,2,3}_table_adj (_nufft_table.pyx:
  * 120-212, 328-441, 574-704) and the CUDA kern_adj launch (_nufft.py:1166-1189).
  * The output grids are zeroed first, as the C kernels do (template.c:336-337).
+ * Points, workspace and fallback as mi355nufft_table1_forward.
  */
 int mi355nufft_table1_adjoint(int ndim, const int64_t *Kd, const int32_t *Jd,
                               int32_t L, int precision, int complex_kernel,

@@ -257,6 +257,44 @@ inline int fixed_j(int ndim, const int32_t* Jd) {
 }
 
 
+// The bin sort of the plan's points (p->d_tm, p->M, p->bg) into its binning
+// storage, in six launches: clear the (bin, bank key) counters, count, sum
+// per bin, scan (subproblem table), scatter (perm), and copy tm and the
+// per-sample phase into sorted order.  With gate.p set every launch does
+// nothing unless the check kernel stamped gate.gen (cached sort).
+template <typename T>
+int launch_bin_sort(const mi355nufft_plan* p, SortGate gate) {
+    const int64_t M = p->M;
+    const int64_t max_sub = max_subproblems(p);
+    const BinState bs = bin_state(p);
+    InterpArgs<T, false> ia;
+    int32_t J[3] = {p->J[0], p->J[1], p->J[2]};
+    const void* dummy[3] = {p->d_tab[0], p->d_tab[0], p->d_tab[0]};
+    fill_interp_args<T, false>(ia, p->ndim, p->K, J, p->L, dummy, p->d_tm, M, 1);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(nblocks((int64_t)p->bg.nbins * kBanks)), dim3(kBlock), 0, p->stream,
+                       bs.bkt, (int64_t)p->bg.nbins * kBanks, 0, gate);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL((bin_count_kernel<T>), dim3(nblocks(M)), dim3(kBlock), 0, p->stream, (const T*)p->d_tm, M,
+                       p->ndim, ia.ax[0], ia.ax[1], ia.ax[2], p->bg, bs, gate);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(bin_sum_kernel, dim3(nblocks(p->bg.nbins)), dim3(kBlock), 0, p->stream, bs, p->bg.nbins,
+                       gate);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, p->stream, bs, p->bg.nbins, p->bg.P, (int)max_sub,
+                       gate);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(bin_scatter_kernel, dim3(nblocks(M)), dim3(kBlock), 0, p->stream, M, bs, p->bg.P, gate);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL((sort_points_kernel<T>), dim3(nblocks(M)), dim3(kBlock), 0, p->stream, (const T*)p->d_tm,
+                       p->has_phase ? (const cx<T>*)p->d_phase : nullptr, M, p->ndim, (const int*)bs.perm,
+                       (T*)p->d_tms, (cx<T>*)p->d_phs, gate);
+    LAUNCH_CHECK();
+    return MI355NUFFT_OK;
+}
+
+// bin geometry of the plan's binning request (defaults per ndim; main TU)
+void setup_bin_geom(mi355nufft_plan* p);
+
 // events for the next launch of kind w (0 gather, 1 scatter) while kernel
 // timing is on, or nulls
 int timing_events(mi355nufft_plan* p, int w, hipEvent_t* start, hipEvent_t* stop);

@@ -5,6 +5,8 @@
 // enqueued on the plan's stream.
 #include <hipcub/hipcub.hpp>
 
+#include <mutex>
+
 #include "internal.hpp"
 #include "kernels_coil.hpp"
 
@@ -690,6 +692,10 @@ int fft_impl(mi355nufft_plan* p, void* grid, int32_t nreps, int direction) {
     return MI355NUFFT_OK;
 }
 
+}  // namespace
+
+namespace mi355host {
+
 // Default bin geometry (autotuned on golden-angle radial and kooshball
 // trajectories): 2-D 16x16 with P = 256 points per subproblem, 3-D 16x8x8
 // with P = 1024, 1-D 512 with P = 512.  Requested sizes override.
@@ -756,6 +762,10 @@ void setup_bin_geom(mi355nufft_plan* p) {
     }
 }
 
+}  // namespace mi355host
+
+namespace {
+
 // LDS of a tiled launch: the gather stages complex T, the scatter
 // accumulates complex float64 (see spread_tiled_kernel); the larger of the two
 // must fit the budget since both share the binning.
@@ -795,23 +805,6 @@ int bin_points_impl(mi355nufft_plan* p) {
         return fail(MI355NUFFT_EINVAL, "too many bins (%d) for the bin sort", p->bg.nbins);
     RET_IF(grow(p, &p->d_bin_b, p->bin_b_cap, (int64_t)p->bg.nbins, 3 + kBanks));
     RET_IF(grow(p, &p->d_sub, p->bin_s_cap, max_sub, 3));
-    BinState bs = bin_state(p);
-    InterpArgs<T, false> ia;
-    int32_t J[3] = {p->J[0], p->J[1], p->J[2]};
-    const void* dummy[3] = {p->d_tab[0], p->d_tab[0], p->d_tab[0]};
-    fill_interp_args<T, false>(ia, p->ndim, p->K, J, p->L, dummy, p->d_tm, M, 1);
-    hipLaunchKernelGGL(fill_int_kernel, dim3(nblocks((int64_t)p->bg.nbins * kBanks)), dim3(kBlock), 0, p->stream,
-                       bs.bkt, (int64_t)p->bg.nbins * kBanks, 0);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL((bin_count_kernel<T>), dim3(nblocks(M)), dim3(kBlock), 0, p->stream, (const T*)p->d_tm, M,
-                       p->ndim, ia.ax[0], ia.ax[1], ia.ax[2], p->bg, bs);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(bin_sum_kernel, dim3(nblocks(p->bg.nbins)), dim3(kBlock), 0, p->stream, bs, p->bg.nbins);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, p->stream, bs, p->bg.nbins, p->bg.P, (int)max_sub);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(bin_scatter_kernel, dim3(nblocks(M)), dim3(kBlock), 0, p->stream, M, bs, p->bg.P);
-    LAUNCH_CHECK();
     if (M > p->sorted_cap) {
         if (p->d_tms) {
             HIP_TRY(hipStreamSynchronize(p->stream));
@@ -825,10 +818,7 @@ int bin_points_impl(mi355nufft_plan* p) {
             return fail(MI355NUFFT_ENOMEM, "sorted points storage for M=%lld", (long long)M);
         p->sorted_cap = M;
     }
-    hipLaunchKernelGGL((sort_points_kernel<T>), dim3(nblocks(M)), dim3(kBlock), 0, p->stream, (const T*)p->d_tm,
-                       p->has_phase ? (const cx<T>*)p->d_phase : nullptr, M, p->ndim, (const int*)bs.perm,
-                       (T*)p->d_tms, (cx<T>*)p->d_phs);
-    LAUNCH_CHECK();
+    RET_IF(launch_bin_sort<T>(p, SortGate{nullptr, 0}));
     p->binned = 1;
     return MI355NUFFT_OK;
 }
@@ -1000,6 +990,218 @@ int need_points(const mi355nufft_plan* p) {
     if (p->M == 0 && !p->d_tm) return fail(MI355NUFFT_ESTATE, "mi355nufft_set_points was not called");
     return MI355NUFFT_OK;
 }
+
+
+// ------------------------------------------------------- raw table1 entry points
+// mi355nufft_table1_forward / _adjoint take the reference's unsorted points
+// (tm, table pointers, grid) on every call, like the C kernels they replace.
+// They run the tiled kernels on a bin sort of tm that is cached: a check
+// kernel compares tm with the copy the cached sort was made from and stamps
+// a per-call generation number when anything differs; the six sort launches
+// are gated on that stamp, so an unchanged tm costs one compare pass and
+// six empty launches, and a changed one (even in place, at the same address)
+// is re-sorted within the same call.  The host never reads anything back.
+// Geometries whose scatter tile does not fit the LDS budget run the
+// global-memory kernels, which need no workspace.
+
+struct RawKey {
+    const void* tm = nullptr;
+    int64_t M = -1;
+    int ndim = 0;
+    int precision = -1;
+    int64_t K[3] = {0, 0, 0};
+    int J[3] = {0, 0, 0};
+    bool same(const RawKey& o) const {
+        if (tm != o.tm || M != o.M || ndim != o.ndim || precision != o.precision) return false;
+        for (int d = 0; d < 3; ++d)
+            if (K[d] != o.K[d] || J[d] != o.J[d]) return false;
+        return true;
+    }
+};
+
+// Byte offsets of one workspace: gate word, copy of tm, bin arrays
+// (bin_of, rank, perm), per-bin arrays, subproblem table, sorted tm.
+struct RawLayout {
+    size_t gate, tm_copy, bin_i, bin_b, sub, tms, total;
+};
+
+RawLayout raw_layout(int64_t M, int ndim, size_t rs, int64_t nbins, int64_t max_sub) {
+    auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    RawLayout L;
+    size_t o = 0;
+    L.gate = o;
+    o = up(o + 256);
+    L.tm_copy = o;
+    o = up(o + (size_t)ndim * (size_t)M * rs);
+    L.bin_i = o;
+    o = up(o + (size_t)3 * (size_t)M * sizeof(int));
+    L.bin_b = o;
+    o = up(o + (size_t)(kBanks + 3) * (size_t)nbins * sizeof(int));
+    L.sub = o;
+    o = up(o + (size_t)(3 * max_sub + 1) * sizeof(int));
+    L.tms = o;
+    o = up(o + (size_t)ndim * (size_t)M * rs);
+    L.total = o;
+    return L;
+}
+
+// One workspace with the state of its cached sort.
+struct RawWork {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    unsigned char* base = nullptr;
+    size_t bytes = 0;
+    RawKey key;            // points and geometry of the cached sort
+    size_t key_total = 0;  // layout it was made in
+    unsigned gen = 0;      // last generation number used
+};
+
+constexpr int kMaxDevices = 64;
+// library-owned workspaces, one per (device, stream) in use, at most this
+// many; beyond that the raw entry points use the global-memory kernels
+constexpr size_t kMaxRawWorkspaces = 16;
+
+std::mutex g_raw_mu;
+std::vector<RawWork> g_raw_work;
+int g_lds_limit[kMaxDevices] = {0};
+
+size_t device_lds_limit(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return 65536;
+    if (!g_lds_limit[dev]) {
+        int lds = 0;
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || lds <= 0)
+            lds = 65536;
+        g_lds_limit[dev] = lds;
+    }
+    return (size_t)g_lds_limit[dev];
+}
+
+// A plan describing the raw call: the caller's tables and tm, the default
+// binning, no phases and no ortho scaling (phase_before / phase_after are
+// applied by the reference's Python around these kernels).  Its binning
+// storage points into the workspace (set by raw_bind).
+void raw_plan(mi355nufft_plan& q, int ndim, const int64_t* Kd, const int32_t* Jd, int32_t L, int precision,
+              int complex_kernel, const void* const* h, const void* tm, int64_t M, hipStream_t s, size_t lds) {
+    q.ndim = ndim;
+    for (int d = 0; d < ndim; ++d) {
+        q.K[d] = Kd[d];
+        q.J[d] = Jd[d];
+        q.nk *= Kd[d];
+        q.d_tab[d] = const_cast<void*>(h[d]);
+    }
+    q.L = L;
+    q.precision = precision;
+    q.phasing = complex_kernel ? 1 : 0;
+    q.fft_phased = 1;  // no phase_before in the staging or the flush
+    q.ortho = 0;
+    q.stream = s;
+    q.M = M;
+    q.d_tm = const_cast<void*>(tm);
+    q.has_phase = 0;
+    q.nframes = 1;
+    q.lds_limit = lds;
+    setup_bin_geom(&q);
+}
+
+RawLayout raw_bind(mi355nufft_plan& q, unsigned char* base) {
+    const RawLayout L = raw_layout(q.M, q.ndim, q.real_size(), q.bg.nbins, max_subproblems(&q));
+    q.d_bin_i = reinterpret_cast<int*>(base + L.bin_i);
+    q.bin_m_cap = q.M;
+    q.d_bin_b = reinterpret_cast<int*>(base + L.bin_b);
+    q.bin_b_cap = q.bg.nbins;
+    q.d_sub = reinterpret_cast<int*>(base + L.sub);
+    q.bin_s_cap = max_subproblems(&q);
+    q.d_tms = base + L.tms;
+    q.d_phs = nullptr;
+    q.binned = 1;
+    return L;
+}
+
+// Enqueue the cached sort of q's points in workspace w (bound by raw_bind)
+// and the tiled gather / scatter.
+template <typename T>
+int raw_tiled(mi355nufft_plan& q, RawWork& w, const RawLayout& L, bool adjoint, const void* fm_in, void* fm_out,
+              const void* ck_in, void* ck_out, int32_t nreps) {
+    RawKey key;
+    key.tm = q.d_tm;
+    key.M = q.M;
+    key.ndim = q.ndim;
+    key.precision = q.precision;
+    for (int d = 0; d < q.ndim; ++d) {
+        key.K[d] = q.K[d];
+        key.J[d] = q.J[d];
+    }
+    const int force = (!w.key.same(key) || w.key_total != L.total) ? 1 : 0;
+    w.gen = w.gen + 1 ? w.gen + 1 : 1;
+    unsigned* gate = reinterpret_cast<unsigned*>(w.base + L.gate);
+    const int64_t n = (int64_t)q.ndim * q.M;
+    hipLaunchKernelGGL((tm_check_kernel<T>), dim3(nblocks(n)), dim3(kBlock), 0, q.stream, (const T*)q.d_tm,
+                       (T*)(w.base + L.tm_copy), n, gate, w.gen, force);
+    LAUNCH_CHECK();
+    // a failed launch below leaves the key unset, so the next call re-sorts
+    w.key = RawKey();
+    RET_IF(launch_bin_sort<T>(&q, force ? SortGate{nullptr, 0} : SortGate{gate, w.gen}));
+    w.key = key;
+    w.key_total = L.total;
+    return q.phasing ? run_tiled<T, true>(&q, adjoint, fm_in, fm_out, ck_in, ck_out, nreps)
+                     : run_tiled<T, false>(&q, adjoint, fm_in, fm_out, ck_in, ck_out, nreps);
+}
+
+// The workspace of (device, stream), grown to `need` bytes; null when the
+// table of workspaces is full or the allocation fails (callers then use the
+// global-memory kernels).
+RawWork* raw_workspace(int dev, hipStream_t s, size_t need) {
+    RawWork* w = nullptr;
+    for (auto& e : g_raw_work)
+        if (e.device == dev && e.stream == s) w = &e;
+    if (!w) {
+        if (g_raw_work.size() >= kMaxRawWorkspaces) return nullptr;
+        g_raw_work.emplace_back();
+        w = &g_raw_work.back();
+        w->device = dev;
+        w->stream = s;
+    }
+    if (need > w->bytes) {
+        if (w->base) {
+            if (hipStreamSynchronize(s) != hipSuccess) return nullptr;
+            (void)hipFree(w->base);
+            w->base = nullptr;
+            w->bytes = 0;
+        }
+        const size_t sz = need + need / 4;
+        if (hipMalloc((void**)&w->base, sz) != hipSuccess) {
+            w->base = nullptr;
+            return nullptr;
+        }
+        w->bytes = sz;
+        w->key = RawKey();  // fresh storage holds no sort
+    }
+    return w;
+}
+
+template <typename T>
+int raw_table1(bool adjoint, int ndim, const int64_t* Kd, const int32_t* Jd, int32_t L, int complex_kernel,
+               const void* fm_in, void* fm_out, const void* ck_in, void* ck_out, const void* const* h,
+               const void* tm, int64_t M, int32_t nreps, hipStream_t s) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (M > 0) {
+        mi355nufft_plan q;
+        raw_plan(q, ndim, Kd, Jd, L, sizeof(T) == 8 ? 1 : 0, complex_kernel, h, tm, M, s, device_lds_limit(dev));
+        if (tile_bytes(&q) <= q.lds_limit && (int64_t)q.bg.nbins * kBanks < ((int64_t)1 << 31)) {
+            std::lock_guard<std::mutex> lock(g_raw_mu);
+            const RawLayout need = raw_layout(M, ndim, sizeof(T), q.bg.nbins, max_subproblems(&q));
+            RawWork* w = raw_workspace(dev, s, need.total);
+            if (w) {
+                const RawLayout Lb = raw_bind(q, w->base);
+                return raw_tiled<T>(q, *w, Lb, adjoint, fm_in, fm_out, ck_in, ck_out, nreps);
+            }
+        }
+    }
+    return run_table1<T>(adjoint, ndim, Kd, Jd, L, complex_kernel, fm_in, fm_out, ck_in, ck_out, h, tm, M, nreps,
+                         nullptr, s);
+}
+
 
 }  // namespace
 
@@ -1528,10 +1730,10 @@ int mi355nufft_table1_forward(int ndim, const int64_t* Kd, const int32_t* Jd, in
         if (!h[d]) return fail(MI355NUFFT_EINVAL, "h[%d] is null", d);
     if (M > 0 && (!ck || !tm || !fm)) return fail(MI355NUFFT_EINVAL, "ck, tm and fm are required");
     hipStream_t s = (hipStream_t)stream;
-    return precision ? run_table1<double>(false, ndim, Kd, Jd, L, complex_kernel, nullptr, fm, ck, nullptr, h, tm,
-                                          M, nreps, nullptr, s)
-                     : run_table1<float>(false, ndim, Kd, Jd, L, complex_kernel, nullptr, fm, ck, nullptr, h, tm,
-                                         M, nreps, nullptr, s);
+    return precision ? raw_table1<double>(false, ndim, Kd, Jd, L, complex_kernel, nullptr, fm, ck, nullptr, h, tm,
+                                          M, nreps, s)
+                     : raw_table1<float>(false, ndim, Kd, Jd, L, complex_kernel, nullptr, fm, ck, nullptr, h, tm,
+                                         M, nreps, s);
 }
 
 int mi355nufft_table1_adjoint(int ndim, const int64_t* Kd, const int32_t* Jd, int32_t L, int precision,
@@ -1550,13 +1752,13 @@ int mi355nufft_table1_adjoint(int ndim, const int64_t* Kd, const int32_t* Jd, in
     for (int d = 0; d < ndim; ++d) nk *= Kd[d];
     if (precision) {
         RET_IF(zero_grid<double>(ck, nk * nreps, s));
-        return run_table1<double>(true, ndim, Kd, Jd, L, complex_kernel, fm, nullptr, nullptr, ck, h, tm, M, nreps,
-                                  nullptr, s);
+        return raw_table1<double>(true, ndim, Kd, Jd, L, complex_kernel, fm, nullptr, nullptr, ck, h, tm, M, nreps,
+                                  s);
     }
     RET_IF(zero_grid<float>(ck, nk * nreps, s));
-    return run_table1<float>(true, ndim, Kd, Jd, L, complex_kernel, fm, nullptr, nullptr, ck, h, tm, M, nreps,
-                             nullptr, s);
+    return raw_table1<float>(true, ndim, Kd, Jd, L, complex_kernel, fm, nullptr, nullptr, ck, h, tm, M, nreps, s);
 }
+
 
 int mi355nufft_toeplitz_create(mi355nufft_toeplitz** out, int ndim, const int64_t* Nd, int precision,
                                void* stream) {
