@@ -387,6 +387,28 @@ int mi355nufft_table1_adjoint(int ndim, const int64_t *Kd, const int32_t *Jd,
                               const void *tm, int64_t M, void *ck,
                               int32_t nreps, void *stream);
 
+/*
+ * mi355nufft_table1_adjoint that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, result and zeroed output as
+ * mi355nufft_table1_adjoint; the bin sort of tm lives in a static device
+ * array of 64 MiB inside the library (one per GPU, held until the process
+ * ends) instead of a workspace the library allocates.  It needs
+ * ndim*2*sizeof(real) + 12 bytes per point and 140 bytes per bin of the
+ * default binning (2-D complex64: about 2.3 M points); larger inputs are
+ * refused with MI355NUFFT_EINVAL before anything is enqueued.  Every call
+ * on a GPU shares that GPU's array, which keeps the last call's sort (reused
+ * while tm is unchanged, re-made otherwise): calls that overlap on one GPU
+ * (on different streams, or from threads without ordering) race.
+ */
+int mi355nufft_table1_adjoint_noalloc(int ndim, const int64_t *Kd,
+                                      const int32_t *Jd, int32_t L,
+                                      int precision, int complex_kernel,
+                                      const void *fm, const void *const *h,
+                                      const void *tm, int64_t M, void *ck,
+                                      int32_t nreps, void *stream);
+
 /* ---- Toeplitz normal operator ---- */
 
 /*

@@ -76,6 +76,8 @@ _SIGNATURES = {
                                           c_vp, c_i32, c_vp]),
     "mi355nufft_table1_adjoint": (c_int, [c_int, c_i64p, c_i32p, c_i32, c_int, c_int, c_vp, c_vpp, c_vp, c_i64,
                                           c_vp, c_i32, c_vp]),
+    "mi355nufft_table1_adjoint_noalloc": (c_int, [c_int, c_i64p, c_i32p, c_i32, c_int, c_int, c_vp, c_vpp, c_vp,
+                                                  c_i64, c_vp, c_i32, c_vp]),
 }
 
 EXPORTED_SYMBOLS = tuple(_SIGNATURES)

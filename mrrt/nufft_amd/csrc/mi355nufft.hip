@@ -1202,6 +1202,70 @@ int raw_table1(bool adjoint, int ndim, const int64_t* Kd, const int32_t* Jd, int
                          nullptr, s);
 }
 
+// ------------------------------------------------- no-allocation adjoint
+// Scratch of mi355nufft_table1_adjoint_noalloc: a file-scope device array
+// (each GPU holds its own copy, at its own address, for the life of the
+// process).  Host code never uses its name as an address: the address on
+// each device comes from hipGetSymbolAddress, taken once per device with
+// that device current, and only that pointer reaches kernels.
+constexpr size_t kNoallocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+std::mutex g_noalloc_mu;
+void* g_scratch_ptr[kMaxDevices] = {nullptr};
+RawWork g_noalloc_work[kMaxDevices];
+
+int noalloc_scratch(int dev, void** out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(MI355NUFFT_EINVAL, "device %d out of range", dev);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(MI355NUFFT_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+        g_noalloc_work[dev].device = dev;
+        g_noalloc_work[dev].base = (unsigned char*)ptr;
+        g_noalloc_work[dev].bytes = kNoallocScratchBytes;
+    }
+    *out = g_scratch_ptr[dev];
+    return MI355NUFFT_OK;
+}
+
+template <typename T>
+int raw_adjoint_noalloc(int ndim, const int64_t* Kd, const int32_t* Jd, int32_t L, int complex_kernel,
+                        const void* fm, void* ck, const void* const* h, const void* tm, int64_t M, int32_t nreps,
+                        hipStream_t s) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    void* base = nullptr;
+    RET_IF(noalloc_scratch(dev, &base));
+    int64_t nk = 1;
+    for (int d = 0; d < ndim; ++d) nk *= Kd[d];
+    mi355nufft_plan q;
+    bool tiled = false;
+    if (M > 0) {
+        raw_plan(q, ndim, Kd, Jd, L, sizeof(T) == 8 ? 1 : 0, complex_kernel, h, tm, M, s, device_lds_limit(dev));
+        tiled = tile_bytes(&q) <= q.lds_limit && (int64_t)q.bg.nbins * kBanks < ((int64_t)1 << 31);
+    }
+    if (tiled) {
+        const RawLayout need = raw_layout(M, ndim, sizeof(T), q.bg.nbins, max_subproblems(&q));
+        if (need.total > kNoallocScratchBytes)
+            return fail(MI355NUFFT_EINVAL,
+                        "mi355nufft_table1_adjoint_noalloc: %lld points need %zu bytes of scratch, more than the "
+                        "%zu bytes it holds (use mi355nufft_table1_adjoint)",
+                        (long long)M, need.total, kNoallocScratchBytes);
+    }
+    RET_IF(zero_grid<T>(ck, nk * nreps, s));  // the output starts from zero (template.c:336-337)
+    if (M == 0) return MI355NUFFT_OK;
+    if (!tiled)  // the global-memory scatter needs no scratch
+        return run_table1<T>(true, ndim, Kd, Jd, L, complex_kernel, fm, nullptr, nullptr, ck, h, tm, M, nreps,
+                             nullptr, s);
+    RawWork& w = g_noalloc_work[dev];
+    w.stream = s;
+    const RawLayout Lb = raw_bind(q, (unsigned char*)base);
+    return raw_tiled<T>(q, w, Lb, true, fm, nullptr, nullptr, ck, nreps);
+}
 
 }  // namespace
 
@@ -1759,6 +1823,22 @@ int mi355nufft_table1_adjoint(int ndim, const int64_t* Kd, const int32_t* Jd, in
     return raw_table1<float>(true, ndim, Kd, Jd, L, complex_kernel, fm, nullptr, nullptr, ck, h, tm, M, nreps, s);
 }
 
+int mi355nufft_table1_adjoint_noalloc(int ndim, const int64_t* Kd, const int32_t* Jd, int32_t L, int precision,
+                                      int complex_kernel, const void* fm, const void* const* h, const void* tm,
+                                      int64_t M, void* ck, int32_t nreps, void* stream) {
+    g_err.clear();
+    RET_IF(check_geometry(ndim, Kd, Jd, L));
+    RET_IF(check_nreps(nreps));
+    if (precision != 0 && precision != 1) return fail(MI355NUFFT_EINVAL, "precision must be 0 or 1");
+    if (M < 0 || M >= ((int64_t)1 << 31)) return fail(MI355NUFFT_EINVAL, "M must be in [0, 2^31)");
+    if (!h || !ck) return fail(MI355NUFFT_EINVAL, "h and ck are required");
+    for (int d = 0; d < ndim; ++d)
+        if (!h[d]) return fail(MI355NUFFT_EINVAL, "h[%d] is null", d);
+    if (M > 0 && (!fm || !tm)) return fail(MI355NUFFT_EINVAL, "fm and tm are required");
+    hipStream_t s = (hipStream_t)stream;
+    return precision ? raw_adjoint_noalloc<double>(ndim, Kd, Jd, L, complex_kernel, fm, ck, h, tm, M, nreps, s)
+                     : raw_adjoint_noalloc<float>(ndim, Kd, Jd, L, complex_kernel, fm, ck, h, tm, M, nreps, s);
+}
 
 int mi355nufft_toeplitz_create(mi355nufft_toeplitz** out, int ndim, const int64_t* Nd, int precision,
                                void* stream) {
