@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MI355NUFFT_ABI_VERSION 1
+#define MI355NUFFT_ABI_VERSION 2
 
 enum {
     MI355NUFFT_OK = 0,
@@ -125,8 +125,9 @@ int64_t mi355nufft_num_points(const mi355nufft_plan *plan);
  * Bin geometry and kernel choice for the interpolation stages (no counterpart
  * in the reference, whose kernels run one thread per sample unsorted).
  *   bin_size    ndim bin sizes in grid points (NULL or 0 entries: defaults
- *               2-D 16x16, 3-D 16x8x8, 1-D 512; the scatter tile is
- *               accumulated in complex float64 and must fit the LDS budget)
+ *               2-D 16x16, 3-D 16x8x8, 1-D 512; the scatter tile, 8 bytes
+ *               per point in fixed point or 16 in float64 (see
+ *               mi355nufft_set_scatter), must fit the LDS budget)
  *   max_points  points per subproblem (0: default 256 in 2-D, 1024 in 3-D,
  *               512 in 1-D)
  *   method      0 auto (tiled when the tile fits in LDS), 1 global-memory
@@ -163,12 +164,27 @@ int mi355nufft_read_timing(mi355nufft_plan *plan, int32_t which, int64_t *launch
                            double *total_ms);
 
 /*
- * Plan description: info[0..17] = ndim, M, bin size (3), points per
+ * Accumulation of the tiled scatter's LDS tile (no counterpart in the
+ * reference, whose scatter accumulates in the operator's precision):
+ *   accumulate  0 auto: 3-D complex64 plans accumulate in 32-bit fixed point
+ *               (per subproblem a power-of-two scale chosen so that no cell
+ *               sum can overflow; relative rounding of order P * 2^-30 per
+ *               cell, independent of the order of the atomics; about 3e-6
+ *               relative L2 on a Kd 256^3 kooshball grid), all others in
+ *               float64.  1: float64.  2: 32-bit fixed point for every
+ *               complex64 plan (complex128 stays float64).
+ * Re-sorts the current points if any (the tile size decides whether the
+ * tiled kernels fit the LDS).
+ */
+int mi355nufft_set_scatter(mi355nufft_plan *plan, int accumulate);
+
+/*
+ * Plan description: info[0..18] = ndim, M, bin size (3), points per
  * subproblem, number of bins, tile points, tiled kernels active (0/1),
  * subproblem slots, method, LDS budget in bytes, number of frames,
  * two-phase scatter active (0/1), threads per tiled workgroup, fused pruned
  * FFT active (0/1), sparse-matrix entries (-1 in table mode), sparse-matrix
- * column pieces.
+ * column pieces, 32-bit fixed-point scatter tile (0/1).
  */
 int mi355nufft_plan_info(const mi355nufft_plan *plan, int64_t *info, int32_t n);
 

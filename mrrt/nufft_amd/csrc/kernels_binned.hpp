@@ -242,6 +242,7 @@ struct TiledArgs {
     const cx<T>* phs;       // per-sample phase in sorted order, or null
     int rc;                 // gather: repetitions staged together (LDS holds rc tiles)
     int own_ok;             // scatter: every K_d is a multiple of B_d (see spread_tiled_kernel)
+    float cmax;             // scatter: > 0 accumulates in fixed point; bound of |coefficient product|
 };
 
 // Copy tm and the per-sample phase into sorted (bin) order, so the tiled
@@ -531,6 +532,69 @@ __device__ __forceinline__ void scatter_taps(double* tre, double* tim, const Poi
     }
 }
 
+// Fixed-point scatter taps (complex64): the tap values of a sample already
+// scaled to fixed-point units, rounded to the nearest integer and added to a
+// 32-bit integer LDS tile (ds_add_u32).  Same taps, same products and the
+// same nesting as scatter_taps.
+template <int NDIM, bool CK, int JMAX, bool FIXED>
+__device__ __forceinline__ void scatter_taps_fx(int* tre, int* tim, const PointTaps<float, CK, JMAX>& pt, float2 f,
+                                                const AxisGeom* ax, int T0, int T1) {
+    if constexpr (NDIM == 1) {
+#pragma unroll
+        for (int j1 = 0; j1 < JMAX; ++j1)
+            if (FIXED || j1 < ax[0].J) {
+                float vr, vi;
+                cmul_conj<float, CK>(vr, vi, pt.cr[0][j1], pt.ci[0][j1], f.x, f.y);
+                atomicAdd(tre + pt.l[0] + j1, __float2int_rn(vr));
+                atomicAdd(tim + pt.l[0] + j1, __float2int_rn(vi));
+            }
+    } else {
+        auto rows = [&](float v3r, float v3i, int plane) {
+#pragma unroll
+            for (int j2 = 0; j2 < JMAX; ++j2)
+                if (FIXED || j2 < ax[1].J) {
+                    float v2r, v2i;
+                    cmul_conj<float, CK>(v2r, v2i, pt.cr[1][j2], pt.ci[1][j2], v3r, v3i);
+                    const int row = (plane + pt.l[1] + j2) * T0 + pt.l[0];
+#pragma unroll
+                    for (int j1 = 0; j1 < JMAX; ++j1)
+                        if (FIXED || j1 < ax[0].J) {
+                            float vr, vi;
+                            cmul_conj<float, CK>(vr, vi, pt.cr[0][j1], pt.ci[0][j1], v2r, v2i);
+                            atomicAdd(tre + row + j1, __float2int_rn(vr));
+                            atomicAdd(tim + row + j1, __float2int_rn(vi));
+                        }
+                }
+        };
+        if constexpr (NDIM == 2) {
+            rows(f.x, f.y, 0);
+        } else {
+#pragma unroll FIXED ? JMAX : 1
+            for (int j3 = 0; j3 < JMAX; ++j3)
+                if (FIXED || j3 < ax[2].J) {
+                    float v3r, v3i;
+                    cmul_conj<float, CK>(v3r, v3i, pt.cr[2][j3], pt.ci[2][j3], f.x, f.y);
+                    rows(v3r, v3i, (pt.l[2] + j3) * T1);
+                }
+        }
+    }
+}
+
+// Sum of v over the workgroup, the same value in every thread (wave64
+// butterfly, then the per-wave partial sums in a fixed order).  red holds one
+// float per wave; the leading barrier protects it from a previous use.
+__device__ __forceinline__ float block_sum(float v, float* red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float t = 0.f;
+    const int nw = blockDim.x >> 6;
+    for (int w = 0; w < nw; ++w) t += red[w];
+    return t;
+}
+
 // MULTI: repetitions (coils) are processed a.rc at a time: a.rc tiles are
 // staged, and each point's coefficients are computed once for all of them.
 template <typename T, int NDIM, bool CK, int JMAX, bool FIXED, bool MULTI>
@@ -585,19 +649,31 @@ __device__ __forceinline__ cx<T> sample_times_conj_phase(const InterpArgs<T, CK>
     return mkcx<T>(f.x * pt.ph.x + f.y * pt.ph.y, f.y * pt.ph.x - f.x * pt.ph.y);
 }
 
-// One repetition at a time: staging several repetitions' float64 tiles was
-// measured no faster (the LDS atomics dominate) and costs occupancy.
-template <typename T, int NDIM, bool CK, int JMAX, bool FIXED>
+// One repetition at a time: staging several repetitions' tiles was measured
+// no faster (the LDS atomics dominate) and costs occupancy.
+//
+// FXP (complex64 only): the tile accumulates in 32-bit fixed point.  On
+// gfx950 an LDS float64 atomic add at scattered addresses costs about 21
+// CU-cycles per wave64 instruction and a 32-bit integer one about 7 (a
+// float32 one about 194), and the LDS atomics are most of the scatter's time.
+// Per subproblem and repetition the taps are scaled by 2^sh, sh = 30 - e with
+// 2^(e-1) <= A * cmax < 2^e, A = sum over its samples of |Re y| + |Im y| and
+// cmax bounding every coefficient product: every tap and every cell sum stays
+// below 2^30 + P in magnitude (no overflow), and each tap is rounded to a
+// step of at most 2^-29 A cmax.  The scale is a power of two, so scaling and
+// unscaling are exact, and the integer sums do not depend on the order of
+// the atomics.  Non-finite samples make the whole tile NaN.  Without FXP
+// (and for complex128) the tile accumulates in float64.
+template <typename T, int NDIM, bool CK, int JMAX, bool FIXED, bool FXP>
 __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
+    static_assert(!FXP || sizeof(T) == 4, "fixed-point accumulation is for complex64");
     extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ float red[4];  // block_sum, at most 256 threads
+    using Acc = typename std::conditional<FXP, int, double>::type;
     const InterpArgs<T, CK>& ia = a.ia;
     const BinGeom& bg = a.bg;
-    // The tile accumulates in float64 for both precisions: on gfx950 an LDS
-    // float32 atomic add costs ~20x the cycles of a float64 one (measured
-    // ds_add_f32 vs ds_add_f64 throughput), and the wider sum is also more
-    // accurate than the reference's float32 sum.
-    double* tre = reinterpret_cast<double*>(smem);
-    double* tim = tre + bg.tile;
+    Acc* tre = reinterpret_cast<Acc*>(smem);
+    Acc* tim = tre + bg.tile;
     const AxisGeom ax[3] = {ia.ax[0], ia.ax[1], ia.ax[2]};
     const int T0 = bg.T[0], T1 = bg.T[1];
     const int tid = threadIdx.x;
@@ -607,20 +683,64 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
         load_sub(bg, a.bs, NDIM, sid, s);
         for (int r = 0; r < ia.nreps; ++r) {
             for (int e = tid; e < bg.tile; e += blockDim.x) {
-                tre[e] = 0.0;
-                tim[e] = 0.0;
+                tre[e] = 0;
+                tim[e] = 0;
+            }
+            int sh = 0;        // FXP: tap values are scaled by 2^sh
+            bool bad = false;  // FXP: a non-finite sample
+            if constexpr (FXP) {
+                float part = 0.f;
+                for (int i = s.start + tid; i < s.end; i += blockDim.x) {
+                    const int64_t m = a.bs.perm[i];
+                    const cx<T> y = ia.in[(int64_t)r * ia.M + m];
+                    part += fabsf(y.x) + fabsf(y.y);  // |f| = |y| per component bound (|phase| = 1)
+                }
+                const float A = block_sum(part, red) * a.cmax * 1.0001f;
+                bad = !(A <= 3.0e38f);
+                int e = 0;
+                (void)frexpf(A, &e);
+                sh = 30 - e;
             }
             __syncthreads();
             for (int i = s.start + tid; i < s.end; i += blockDim.x) {
                 PointTaps<T, CK, JMAX> pt;
                 point_taps<T, NDIM, CK, JMAX, FIXED>(a, ax, s, i, pt);
-                scatter_taps<T, NDIM, CK, JMAX, FIXED>(tre, tim, pt, sample_times_conj_phase<T, CK, JMAX>(ia, r, pt, a.phs),
-                                                      ax, T0, T1);
+                cx<T> f = sample_times_conj_phase<T, CK, JMAX>(ia, r, pt, a.phs);
+                if constexpr (FXP) {
+                    if (bad) continue;
+                    f.x = ldexpf(f.x, sh);
+                    f.y = ldexpf(f.y, sh);
+                    scatter_taps_fx<NDIM, CK, JMAX, FIXED>(tre, tim, pt, f, ax, T0, T1);
+                } else {
+                    scatter_taps<T, NDIM, CK, JMAX, FIXED>(tre, tim, pt, f, ax, T0, T1);
+                }
             }
             __syncthreads();
+            auto value = [&](int e, T& vr, T& vi) {  // false: an empty cell
+                if constexpr (FXP) {
+                    const int ir = tre[e], ii = tim[e];
+                    if (bad) {
+                        vr = vi = __int_as_float(0x7fc00000);
+                        return true;
+                    }
+                    if (ir == 0 && ii == 0) return false;
+                    vr = ldexpf((float)ir, -sh);
+                    vi = ldexpf((float)ii, -sh);
+                } else {
+                    const double ar = tre[e], ai = tim[e];
+                    if (ar == 0.0 && ai == 0.0) return false;
+                    vr = (T)ar;
+                    vi = (T)ai;
+                }
+                return true;
+            };
             if (a.tiles) {  // two-phase: write the whole tile to its slot, no phase
                 cx<T>* __restrict__ slot = a.tiles + ((int64_t)sid * ia.nreps + r) * bg.tile;
-                for (int e = tid; e < bg.tile; e += blockDim.x) slot[e] = mkcx<T>((T)tre[e], (T)tim[e]);
+                for (int e = tid; e < bg.tile; e += blockDim.x) {
+                    T vr = 0, vi = 0;
+                    (void)value(e, vr, vi);
+                    slot[e] = mkcx<T>(vr, vi);
+                }
                 __syncthreads();
                 continue;
             }
@@ -632,9 +752,8 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
             // zero there), everything else with atomics.
             const bool own = a.own_ok && s.solo;
             for (int e = tid; e < bg.tile; e += blockDim.x) {
-                const double ar = tre[e], ai = tim[e];
-                if (ar == 0.0 && ai == 0.0) continue;
-                T vr = (T)ar, vi = (T)ai;
+                T vr, vi;
+                if (!value(e, vr, vi)) continue;
                 int g0, g1, g2, e0, e1, e2;
                 const int64_t gi = tile_to_grid<NDIM>(e, bg, s, ax, g0, g1, g2, e0, e1, e2);
                 if (a.use_pb) {  // * conj(phase_before)

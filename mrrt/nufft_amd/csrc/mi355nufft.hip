@@ -799,12 +799,11 @@ int bin_points_impl(mi355nufft_plan* p) {
         return MI355NUFFT_OK;
     }
     const int64_t M = p->M;
-    const int64_t max_sub = max_subproblems(p);
     RET_IF(grow(p, &p->d_bin_i, p->bin_m_cap, M, 3));
     if ((int64_t)p->bg.nbins * kBanks >= ((int64_t)1 << 31))
         return fail(MI355NUFFT_EINVAL, "too many bins (%d) for the bin sort", p->bg.nbins);
     RET_IF(grow(p, &p->d_bin_b, p->bin_b_cap, (int64_t)p->bg.nbins, 3 + kBanks));
-    RET_IF(grow(p, &p->d_sub, p->bin_s_cap, max_sub, 3));
+    RET_IF(grow(p, &p->d_sub, p->bin_s_cap, max_subproblems(p), 3));
     if (M > p->sorted_cap) {
         if (p->d_tms) {
             HIP_TRY(hipStreamSynchronize(p->stream));
@@ -1314,6 +1313,27 @@ int mi355nufft_plan_create(mi355nufft_plan** plan, int ndim, const int64_t* Nd, 
             hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
             p->lds_limit = (size_t)lds;  // 160 KiB per workgroup on gfx950
     }
+    // prod_d max |h_d|: bounds every coefficient product (each coefficient
+    // interpolates two neighbouring table entries), for the fixed-point scatter
+    p->tab_cmax = 1.0;
+    for (int d = 0; d < ndim; ++d) {
+        const int64_t n = (int64_t)Jd[d] * L + 1;
+        double mx = 0.0;
+        for (int64_t i = 0; i < n; ++i) {
+            double v;
+            if (phasing) {
+                const double re = precision ? ((const double*)tables[d])[2 * i] : ((const float*)tables[d])[2 * i];
+                const double im =
+                    precision ? ((const double*)tables[d])[2 * i + 1] : ((const float*)tables[d])[2 * i + 1];
+                v = std::sqrt(re * re + im * im);
+            } else {
+                v = std::fabs(precision ? ((const double*)tables[d])[i] : ((const float*)tables[d])[i]);
+            }
+            if (!(v <= mx)) mx = v;  // NaN propagates
+        }
+        p->tab_cmax *= mx;
+    }
+    if (!(p->tab_cmax > 0.0 && p->tab_cmax < 1e30)) p->tab_cmax = 0.0;  // unusable: float64 scatter tile
     for (int d = 0; d < ndim; ++d) {
         const size_t tab_bytes = (size_t)(Jd[d] * L + 1) * (phasing ? p->cx_size() : p->real_size());
         if (hipMalloc(&p->d_tab[d], tab_bytes) != hipSuccess ||
@@ -1515,11 +1535,12 @@ int mi355nufft_plan_info(const mi355nufft_plan* p, int64_t* info, int32_t n) {
     g_err.clear();
     RET_IF(check_plan(p));
     if (!info || n < 1) return fail(MI355NUFFT_EINVAL, "info buffer required");
-    const int64_t v[18] = {p->ndim, p->M, p->bg.B[0], p->bg.B[1], p->bg.B[2], p->bg.P, p->bg.nbins, p->bg.tile,
+    const int64_t v[19] = {p->ndim, p->M, p->bg.B[0], p->bg.B[1], p->bg.B[2], p->bg.P, p->bg.nbins, p->bg.tile,
                            use_tiled(p) ? 1 : 0, p->binned ? max_subproblems(p) : 0, p->method, (int64_t)p->lds_limit,
                            p->nframes, (use_tiled(p) && use_two_phase(p, 1)) ? 1 : 0, p->bg_threads,
-                           use_fused_fft(p) ? 1 : 0, p->sp.on ? p->sp.nnz : -1, p->sp.on ? p->sp.npieces : 0};
-    for (int i = 0; i < n && i < 18; ++i) info[i] = v[i];
+                           use_fused_fft(p) ? 1 : 0, p->sp.on ? p->sp.nnz : -1, p->sp.on ? p->sp.npieces : 0,
+                           use_fxp(p) ? 1 : 0};
+    for (int i = 0; i < n && i < 19; ++i) info[i] = v[i];
     return MI355NUFFT_OK;
 }
 
@@ -1695,6 +1716,17 @@ int mi355nufft_weight_samples(int precision, const void* y, const void* w, int64
         hipLaunchKernelGGL(weight_samples_kernel<float>, grid, dim3(kBlock), 0, (hipStream_t)stream,
                            (const cx<float>*)y, (const float*)w, M, (cx<float>*)out);
     LAUNCH_CHECK();
+    return MI355NUFFT_OK;
+}
+
+int mi355nufft_set_scatter(mi355nufft_plan* p, int accumulate) {
+    g_err.clear();
+    RET_IF(check_plan(p));
+    if (accumulate < 0 || accumulate > 2)
+        return fail(MI355NUFFT_EINVAL, "accumulate must be 0 (auto), 1 (float64) or 2 (32-bit fixed point)");
+    p->scatter_acc = accumulate;
+    // the tile size changes, and with it whether the tiled kernels fit
+    if (p->d_tm && p->M > 0) return bin_points(p);
     return MI355NUFFT_OK;
 }
 

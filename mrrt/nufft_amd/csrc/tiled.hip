@@ -13,7 +13,9 @@ namespace mi355host {
 template <typename T, bool CK, int NDIM, int JMAX, bool FIXED>
 void launch_tiled(bool adjoint, const TiledArgs<T, CK>& a, unsigned nsub, unsigned threads, size_t lds,
                   hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
-    auto spread = spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED>;
+    auto spread = spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED, false>;
+    if constexpr (sizeof(T) == 4)
+        if (a.cmax > 0.f) spread = spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED, true>;
     auto interp = a.rc > 1 ? interp_tiled_kernel<T, NDIM, CK, JMAX, FIXED, true>
                            : interp_tiled_kernel<T, NDIM, CK, JMAX, FIXED, false>;
     if (ev0) {
@@ -50,6 +52,7 @@ int run_tiled(mi355nufft_plan* p, bool adjoint, const void* samples_in, void* sa
     a.tiles = adjoint ? (cx<T>*)p->tiles_target : nullptr;
     a.tms = (const T*)p->d_tms;
     a.phs = p->has_phase ? (const cx<T>*)p->d_phs : nullptr;
+    a.cmax = (adjoint && use_fxp(p)) ? (float)p->tab_cmax : 0.f;  // > 0: fixed-point scatter tile
     a.own_ok = 1;
     for (int d = 0; d < p->ndim; ++d)
         if (p->K[d] % p->bg.B[d] != 0) a.own_ok = 0;

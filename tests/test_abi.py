@@ -35,7 +35,7 @@ def test_header_symbols_exported():
 
 
 def test_abi_version():
-    assert _lib.get_lib().mi355nufft_abi_version() == 1
+    assert _lib.get_lib().mi355nufft_abi_version() == 2
 
 
 def test_single_hip_runtime():

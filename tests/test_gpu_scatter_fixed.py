@@ -1,0 +1,107 @@
+"""The scatter's 32-bit fixed-point LDS tile (complex64): per-subproblem
+power-of-two scales bounded by the samples' magnitudes.
+
+Checked against the oracle's adjoint (nufft_adj restated,
+mrrt/nufft/_nufft.py:1459-1578) and against the float64 tile of the same
+plan: at the parity bar for ordinary inputs, without overflow when every
+sample sits in one grid cell (the bound is tight there), exactly scale-
+covariant over the float range (the scale is a power of two), NaN samples
+propagate, zeros stay zeros.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from mrrt.nufft_amd import NufftBase
+from mrrt.nufft_amd import trajectories as trj
+from oracle import nufft_oracle as O
+from tests.cases import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def dev():
+    assert torch.cuda.is_available(), "GPU tests need a ROCm device"
+    return torch.device("cuda", 0)
+
+
+def _case(ndim):
+    if ndim == 2:
+        Nd, Kd = (40, 48), (80, 96)
+        om = trj.golden_angle_2d(60, 80)
+    else:
+        Nd, Kd = (24, 20, 16), (48, 40, 32)
+        om = trj.kooshball_3d(400, 48, seed=4)
+    return Nd, Kd, om
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+@pytest.mark.parametrize("phasing", ["real", "complex"])
+def test_fixed_point_vs_oracle_and_float64(ndim, phasing):
+    Nd, Kd, om = _case(ndim)
+    A = NufftBase(Nd, om, Jd=6, Kd=Kd, Ld=512, phasing=phasing, n_shift=np.asarray(Nd) / 2, device=dev())
+    B = O.NufftOracle(Nd, om, Jd=6, Kd=Kd, Ld=512, phasing=phasing, n_shift=np.asarray(Nd) / 2)
+    rng = np.random.default_rng(ndim)
+    y = (rng.standard_normal((om.shape[0], 3)) + 1j * rng.standard_normal((om.shape[0], 3))).astype(np.complex64)
+    A.set_scatter("fixed32")
+    assert A.plan_info()["fixed_point_scatter"]
+    xf = A.adj(y)
+    A.set_scatter("float64")
+    assert not A.plan_info()["fixed_point_scatter"]
+    xd = A.adj(y)
+    xo = B.adj(y)
+    assert rel_l2(xf, xo) <= 1e-5
+    assert rel_l2(xd, xo) <= 1e-5
+    assert rel_l2(xf, xd) <= 5e-6
+
+
+def test_default_accumulation_per_dimension():
+    """3-D complex64 plans default to the fixed-point tile, 2-D and
+    complex128 plans to float64."""
+    for ndim, precision, want in ((2, "single", False), (3, "single", True), (3, "double", False)):
+        Nd, Kd, om = _case(ndim)
+        A = NufftBase(Nd, om, Jd=6, Kd=Kd, Ld=512, precision=precision, device=dev())
+        assert A.plan_info()["fixed_point_scatter"] == want, (ndim, precision)
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_fixed_point_all_samples_in_one_cell(ndim):
+    """5,000 samples of one grid cell with equal values: every cell sum is at
+    its bound (the tile must not overflow), and identical taps round
+    identically, so their rounding adds coherently (the worst case of the
+    fixed-point error: about 5e-6 by the bound, within the parity bar)."""
+    Nd, Kd, _ = _case(ndim)
+    om = np.tile(np.full((1, ndim), 0.1234), (5000, 1)) + 1e-9 * np.arange(5000)[:, None]
+    A = NufftBase(Nd, om, Jd=6, Kd=Kd, Ld=512, device=dev())
+    y = np.full(5000, 1.0 + 0.5j, np.complex64)
+    A.set_scatter("fixed32")
+    xf = A.adj(y)
+    A.set_scatter("float64")
+    xd = A.adj(y)
+    assert np.all(np.isfinite(xf))
+    assert rel_l2(xf, xd) <= 1e-5
+
+
+def test_fixed_point_range_nan_and_zero():
+    """Inputs scaled by 2^-120 .. 2^100 keep the fixed-point tile's accuracy
+    (its scale is a power of two chosen per subproblem): the adjoint equals
+    the float64 tile's at every scale; NaN samples propagate; zeros stay
+    zeros."""
+    Nd, Kd, om = _case(3)
+    A = NufftBase(Nd, om, Jd=6, Kd=Kd, Ld=512, device=dev())
+    rng = np.random.default_rng(9)
+    y = (rng.standard_normal(om.shape[0]) + 1j * rng.standard_normal(om.shape[0])).astype(np.complex64)
+    for s in (1.0, 2.0 ** -100, 2.0 ** 100, 2.0 ** -120):
+        ys = (y * s).astype(np.complex64)
+        A.set_scatter("fixed32")
+        xf = A.adj(ys)
+        A.set_scatter("float64")
+        xd = A.adj(ys)
+        # in complex128: float32 squares of these values would underflow
+        assert rel_l2(xf.astype(np.complex128), xd.astype(np.complex128)) <= 5e-6, s
+    A.set_scatter("fixed32")
+    assert np.count_nonzero(A.adj(np.zeros_like(y))) == 0
+    yn = y.copy()
+    yn[17] = np.nan
+    assert np.isnan(A.adj(yn)).any()
