@@ -12,10 +12,11 @@ reference builds it through a float32 1-D sparse NUFFT
 expressions directly, without forming the sparse matrix.
 """
 
+import warnings
 from math import sqrt
 
 import numpy as np
-from scipy.special import i0
+from scipy.special import gamma, i0, iv, jv
 
 __all__ = [
     "beatty_alpha",
@@ -32,42 +33,77 @@ def beatty_alpha(J, K, N):
     return np.pi * sqrt(J ** 2 / k_n ** 2 * (k_n - 0.5) ** 2 - 0.8)
 
 
-def kaiser_bessel(x, J=6, alpha=None):
-    """Order-0 Kaiser-Bessel window I0(alpha sqrt(1-(2x/J)^2)) / I0(alpha)
-    on |x| < J/2, zero elsewhere (_kaiser_bessel.py:78-149, m = 0)."""
+def kaiser_bessel(x=None, J=6, alpha=None, m=0, K_N=None):
+    """Generalized Kaiser-Bessel window on |x| < J/2, zero elsewhere
+    (_kaiser_bessel.py:78-149; Lewitt, JOSA-A 1990, eq. A1):
+
+        f^m I_m(alpha f) / I_m(alpha),  f = sqrt(1 - (2x/J)^2).
+
+    ``alpha`` defaults to 2.34 J.  A negative non-integer order uses |m| in
+    the Bessel functions, with a warning, as the reference does.  ``K_N``
+    (grid oversampling) is accepted for the reference's signature and not
+    used.  The order-0 window is the one every NUFFT table uses.
+    """
     if alpha is None:
         alpha = 2.34 * J
+    if m < 0 and abs(round(m) - m) > np.finfo(float).eps:
+        warnings.warn("Negative NonInt m=%g\n\t- using modified definition of KB function\n" % m)
+    m_bi = abs(m)
     x = np.asarray(x)
     inside = 2 * np.abs(x) < J
     u = 2 * x[inside] / J
     u *= u
     f = np.sqrt(1 - u)
     out = np.zeros_like(x)
-    out[inside] = i0(alpha * f) / float(i0(alpha))
+    if m_bi == 0:
+        out[inside] = i0(alpha * f) / float(i0(alpha))
+    else:
+        out[inside] = (f ** m * iv(m_bi, alpha * f)) / float(iv(m_bi, alpha))
     return out.real
 
 
-def kaiser_bessel_ft(u, J=6, alpha=None):
-    """1-D Fourier transform of the order-0 KB window (_kaiser_bessel.py:153-226).
+def kaiser_bessel_ft(u, J=6, alpha=None, m=0, d=1):
+    """Fourier transform of the generalized KB window in d dimensions
+    (_kaiser_bessel.py:153-226; Lewitt, JOSA-A 1990, eq. A3).
 
-    Closed form: J sinh(w) / (I0(alpha) w), w = sqrt(alpha^2 - (pi J u)^2);
+    Order 0, d = 1 (every NUFFT's deapodization): the closed form
+    J sinh(w) / (I0(alpha) w), w = sqrt(alpha^2 - (pi J u)^2);
     J sin(z) / (I0(alpha) z), z = sqrt((pi J u)^2 - alpha^2) past the
-    transition; J / I0(alpha) at w = 0.
+    transition; J / I0(alpha) at w = 0.  Otherwise
+    (2 pi)^(d/2) (J/2)^d alpha^m / I_m(alpha) J_nu(z) / z^nu, nu = d/2 + m,
+    z = sqrt((pi J u)^2 - alpha^2) taken complex where the radicand is
+    negative.
     """
-    if alpha is None:
+    if not alpha:
         alpha = 2.34 * J
     u = np.asarray(u, dtype=np.float64)
+    if m == 0 and d == 1:
+        q = (np.pi * J) * u
+        q = alpha * alpha - q * q
+        out = np.empty_like(q)
+        pos = q > 0
+        neg = q < 0
+        w = np.sqrt(q[pos])
+        out[pos] = np.sinh(w) / w
+        z = np.sqrt(-q[neg])
+        out[neg] = np.sin(z) / z
+        out[~(pos | neg)] = 1.0
+        return J * out / i0(alpha)
+    if m < -1:
+        warnings.warn("m=%g < -1 in kaiser_bessel_ft(): validity of the FT formula uncertain" % m)
+    elif m < 0 and abs(round(m) - m) > np.finfo(float).eps:
+        warnings.warn("negative non-integer m=%g in kaiser_bessel_ft(): validity of the FT formula uncertain" % m)
     q = (np.pi * J) * u
-    q = alpha * alpha - q * q
-    out = np.empty_like(q)
-    pos = q > 0
-    neg = q < 0
-    w = np.sqrt(q[pos])
-    out[pos] = np.sinh(w) / w
-    z = np.sqrt(-q[neg])
-    out[neg] = np.sin(z) / z
-    out[~(pos | neg)] = 1.0
-    return J * out / i0(alpha)
+    z = np.lib.scimath.sqrt(q * q - alpha * alpha)
+    nu = d / 2.0 + m
+    const = (2 * np.pi) ** (d / 2.0) * (J / 2.0) ** d * alpha ** m / (iv(m, alpha) if m else i0(alpha))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        y = const * jv(nu, z) / z ** nu
+    # z = 0 (the transition point): the limit of J_nu(z) / z^nu is 2^-nu / Gamma(nu + 1)
+    zero = z == 0
+    if np.any(zero):
+        y = np.where(zero, const * 2.0 ** -nu / gamma(nu + 1), y)
+    return np.real(y)
 
 
 def build_table(N, J, K, L, phasing="real"):

@@ -14,6 +14,7 @@ reference's CPU path) or torch tensors (results stay on the device).
 """
 
 import ctypes
+import threading
 import warnings
 from math import sqrt
 
@@ -112,15 +113,7 @@ class NufftBase(object):
         omega_np, omega_t = self._check_omega(omega)
         if precision == "auto":
             precision = "single" if omega_t.dtype == torch.float32 else "double"
-        if precision == "single":
-            self._cplx_dtype, self._real_dtype = np.dtype(np.complex64), np.dtype(np.float32)
-        elif precision == "double":
-            self._cplx_dtype, self._real_dtype = np.dtype(np.complex128), np.dtype(np.float64)
-        else:
-            raise ValueError("precision must be 'single', 'double' or 'auto'")
-        self._precision = precision
-        self._tcdt = torch.complex64 if precision == "single" else torch.complex128
-        self._trdt = torch.float32 if precision == "single" else torch.float64
+        self._set_dtypes(precision)
 
         if n_shift is None:
             self._n_shift = (0.0,) * self.ndim
@@ -143,30 +136,47 @@ class NufftBase(object):
         self._sn_factors = [np.ascontiguousarray(
             deapod_factors(self._Nd[d], self._Kd[d], self._Jd[d], self.alpha[d], self.n_mid[d]), dtype=np.float64)
             for d in range(self.ndim)]
+        self._lib = _lib.get_lib()
+        self._plan = None
+        self._create_plan()
+        self._set_points(omega_np, omega_t)
+        self.nargin1 = int(np.prod(self._Nd))
+        self.nargout1 = self.M
+
+    def _set_dtypes(self, precision):
+        if precision == "single":
+            self._cplx_dtype, self._real_dtype = np.dtype(np.complex64), np.dtype(np.float32)
+        elif precision == "double":
+            self._cplx_dtype, self._real_dtype = np.dtype(np.complex128), np.dtype(np.float64)
+        else:
+            raise ValueError("precision must be 'single', 'double' or 'auto'")
+        self._precision = precision
+        self._tcdt = torch.complex64 if precision == "single" else torch.complex128
+        self._trdt = torch.float32 if precision == "single" else torch.float64
+
+    def _create_plan(self):
+        """Tables in the operator's precision and the device plan
+        (mi355nufft_plan_create)."""
         tables = []
         for d in range(self.ndim):
-            h = build_table(self._Nd[d], self._Jd[d], self._Kd[d], self._Ld, phasing)
-            h = h.astype(self._cplx_dtype if phasing == "complex" else self._real_dtype)
+            h = build_table(self._Nd[d], self._Jd[d], self._Kd[d], self._Ld, self._phasing)
+            h = h.astype(self._cplx_dtype if self._phasing == "complex" else self._real_dtype)
             tables.append(np.ascontiguousarray(h))
         self._tables_host = tables
-
-        lib = _lib.get_lib()
-        self._lib = lib
-        self._plan = ctypes.c_void_p()
+        self._h_dev_cache = None
+        plan = ctypes.c_void_p()
         tab_ptrs = (ctypes.c_void_p * self.ndim)(*[t.ctypes.data for t in tables])
         sn_ptrs = (ctypes.POINTER(ctypes.c_double) * self.ndim)(
             *[s.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for s in self._sn_factors])
         phase_coef = _lib.dblarr([2 * np.pi / self._Kd[d] * self.n_mid[d] for d in range(self.ndim)])
         with torch.cuda.device(self.device):
             self._stream = torch.cuda.current_stream(self.device).cuda_stream
-            _lib.check(lib.mi355nufft_plan_create(
-                ctypes.byref(self._plan), self.ndim, _lib.i64arr(self._Nd), _lib.i64arr(self._Kd),
-                _lib.i32arr(self._Jd), self._Ld, 0 if precision == "single" else 1,
-                0 if phasing == "real" else 1, 1 if ortho else 0, float(adjoint_scalefactor),
-                tab_ptrs, sn_ptrs, phase_coef if phasing == "real" else None, ctypes.c_void_p(self._stream)))
-        self._set_points(omega_np, omega_t)
-        self.nargin1 = int(np.prod(self._Nd))
-        self.nargout1 = self.M
+            _lib.check(self._lib.mi355nufft_plan_create(
+                ctypes.byref(plan), self.ndim, _lib.i64arr(self._Nd), _lib.i64arr(self._Kd),
+                _lib.i32arr(self._Jd), self._Ld, 0 if self._precision == "single" else 1,
+                0 if self._phasing == "real" else 1, 1 if self.ortho else 0, float(self.adjoint_scalefactor),
+                tab_ptrs, sn_ptrs, phase_coef if self._phasing == "real" else None, ctypes.c_void_p(self._stream)))
+        self._plan = plan
 
     # ------------------------------------------------------------------
     # properties (names of the reference attributes)
@@ -180,7 +190,34 @@ class NufftBase(object):
     # _nufft.py:510-513; the matrix itself lives on the device (rows for the
     # forward, columns for the adjoint), not as a host scipy matrix
     sparse_format = property(lambda self: "CSC" if self._sparse else None)
-    precision = property(lambda self: self._precision)
+
+    @property
+    def precision(self):
+        """'single' or 'double'.  Setting it rebuilds the operator in the new
+        precision (tables, plan, points; _nufft.py:520-545, where the setter
+        re-casts omega, the phases, sn and the tables); binning, scatter and
+        FFT choices return to their defaults."""
+        return self._precision
+
+    @precision.setter
+    def precision(self, precision):
+        if precision == "auto":
+            precision = "single" if self._omega_user.dtype == torch.float32 else "double"
+        if precision == self._precision:
+            return
+        old = (self._precision, self._plan)
+        self._set_dtypes(precision)
+        try:
+            self._create_plan()
+        except Exception:
+            self._set_dtypes(old[0])
+            self._plan = old[1]
+            raise
+        self._sync_stream()
+        self._lib.mi355nufft_plan_destroy(old[1])
+        frames = getattr(self, "_frame_offsets", None)
+        omega_t, omega_np = self._omega_user, self._omega_np
+        self._set_points(omega_np, omega_t, frames)
     n_shift = property(lambda self: self._n_shift)
 
     @property
@@ -299,6 +336,7 @@ class NufftBase(object):
             phase_mode = 0
             dot_single = 0
         self._sync_stream()
+        self._frame_offsets = frame_offsets
         if frame_offsets is None:
             offs, nframes = None, 1
         else:
@@ -374,13 +412,21 @@ class NufftBase(object):
 
     def plan_info(self):
         """dict with the binning state of the plan (see mi355nufft_plan_info)."""
-        buf = (ctypes.c_int64 * 18)()
-        _lib.check(self._lib.mi355nufft_plan_info(self._plan, buf, 18))
+        buf = (ctypes.c_int64 * 19)()
+        _lib.check(self._lib.mi355nufft_plan_info(self._plan, buf, 19))
         v = list(buf)
         return dict(ndim=v[0], M=v[1], bin_size=tuple(v[2:5]), max_points=v[5], nbins=v[6], tile=v[7],
                     tiled=bool(v[8]), subproblem_slots=v[9], method=v[10], lds_limit=v[11], nframes=v[12],
                     two_phase=bool(v[13]), threads=v[14], fused_fft=bool(v[15]),
-                    sparse_nnz=v[16] if v[16] >= 0 else None, sparse_pieces=v[17])
+                    sparse_nnz=v[16] if v[16] >= 0 else None, sparse_pieces=v[17], fixed_point_scatter=bool(v[18]))
+
+    def set_scatter(self, accumulate="auto"):
+        """Accumulation of the scatter's LDS tile: "auto" (32-bit fixed point
+        for 3-D complex64, float64 otherwise), "float64", or "fixed32"
+        (32-bit fixed point for any complex64 plan; see
+        mi355nufft_set_scatter)."""
+        self._sync_stream()
+        _lib.check(self._lib.mi355nufft_set_scatter(self._plan, {"auto": 0, "float64": 1, "fixed32": 2}[accumulate]))
 
     def set_fft(self, method="auto"):
         """FFT of the transforms: "auto" (pruned passes fused with the
@@ -496,18 +542,89 @@ def _numel(x):
     return x.numel() if isinstance(x, torch.Tensor) else np.size(x)
 
 
+class _HostStaging(object):
+    """Pinned host buffers of one device for numpy inputs and outputs,
+    allocated once and grown on demand (a fresh pinned allocation per call
+    costs more than the transfer).  An event recorded after each upload
+    guards the input buffer against being overwritten while the transfer is
+    still in flight; the output buffer is only used under a stream
+    synchronisation.  Results are copied into ordinary numpy arrays, so
+    callers never hold page-locked memory."""
+
+    def __init__(self):
+        self.lock = threading.Lock()
+        self.inbuf = None
+        self.outbuf = None
+        self.in_event = None
+
+    @staticmethod
+    def _grow(buf, nbytes):
+        if buf is None or buf.numel() < nbytes:
+            buf = torch.empty(max(nbytes, 1 << 16), dtype=torch.uint8, pin_memory=True)
+        return buf
+
+    def upload(self, x, device):
+        """numpy x -> device tensor of the same shape and dtype (asynchronous)."""
+        tdt = _TORCH_DTYPE[x.dtype]
+        with self.lock:
+            if self.in_event is not None:
+                self.in_event.synchronize()
+            self.inbuf = self._grow(self.inbuf, x.nbytes)
+            staged = self.inbuf[:x.nbytes].view(tdt).reshape(x.shape)
+            np.copyto(staged.numpy(), x, casting="no")
+            t = staged.to(device, non_blocking=True)
+            if self.in_event is None:
+                self.in_event = torch.cuda.Event()
+            self.in_event.record(torch.cuda.current_stream(device))
+        return t
+
+    def download(self, t):
+        """device tensor -> new ordinary numpy array (waits for the copy)."""
+        nbytes = t.numel() * t.element_size()
+        with self.lock:
+            self.outbuf = self._grow(self.outbuf, nbytes)
+            staged = self.outbuf[:nbytes].view(t.dtype).reshape(t.shape)
+            staged.copy_(t, non_blocking=True)
+            torch.cuda.current_stream(t.device).synchronize()
+            return staged.numpy().copy()
+
+
+_TORCH_DTYPE = {np.dtype(np.complex64): torch.complex64, np.dtype(np.complex128): torch.complex128,
+                np.dtype(np.float32): torch.float32, np.dtype(np.float64): torch.float64}
+_STAGING = {}
+_STAGING_LOCK = threading.Lock()
+
+
+def _staging(device):
+    key = (device.type, device.index if device.index is not None else torch.cuda.current_device())
+    with _STAGING_LOCK:
+        st = _STAGING.get(key)
+        if st is None:
+            st = _STAGING[key] = _HostStaging()
+    return st
+
+
+def _upload_host(x, device):
+    """numpy array -> device tensor of the same shape (floating or complex
+    dtypes; others as float64), through the device's pinned staging buffer."""
+    x = np.asarray(x)
+    if x.dtype not in _TORCH_DTYPE:
+        x = x.astype(np.float64)
+    return _staging(device).upload(x, device)
+
+
 def _to_device(obj, x):
     """Returns (tensor on device with the operator's complex dtype, was_numpy).
 
-    Host arrays go through a pinned staging copy (torch's caching host
-    allocator), so the transfer to the device runs at pinned-memory rate."""
+    Host arrays go through the device's persistent pinned staging buffer,
+    so the transfer runs at pinned-memory rate without a pinned allocation
+    per call."""
     was_numpy = not isinstance(x, torch.Tensor)
     if was_numpy:
         x = np.asarray(x)
         if x.dtype == np.complex128 and obj._precision == "single":
             x = x.astype(np.complex64)
-        t = torch.from_numpy(x) if x.dtype.kind in "fc" else torch.from_numpy(x.astype(np.float64))
-        t = t.pin_memory().to(obj.device, non_blocking=True)
+        t = _upload_host(x, obj.device)
     else:
         t = x.to(obj.device)
     if not t.is_complex():
@@ -519,12 +636,10 @@ def _to_device(obj, x):
 
 
 def _to_output(t, was_numpy):
-    """Host results are copied into a new pinned tensor (synchronous copy)
-    whose memory the returned array keeps alive."""
+    """Device results for numpy callers: copied through the device's pinned
+    staging buffer into a new ordinary numpy array (synchronous)."""
     if was_numpy:
-        out = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
-        out.copy_(t)
-        return out.numpy()
+        return _staging(t.device).download(t)
     return t
 
 
@@ -571,14 +686,25 @@ def _phase_shift_dev(obj):
     return torch.exp(1j * (obj._omega_rdt.double() @ sh))
 
 
-def nufft_forward(obj, x, grid_only=False):
-    """Forward table1 NUFFT (mrrt/nufft/_nufft.py:1275-1397).
+def _check_xp(xp):
+    """The reference's array-module argument: numpy (or None) selects numpy
+    results for numpy inputs, torch keeps device tensors; the transform runs
+    on the GPU either way."""
+    if xp is not None and xp is not np and xp is not torch:
+        raise ValueError("xp must be None, numpy or torch")
 
-    ``grid_only``: ``x`` is the oversampled grid (prod(Kd), reps); only the
-    table interpolation is applied (no FFT, phase_before, ortho or
-    phase_after; the complex kernel's phase_shift is kept, :1086-1095) and
-    the (M, reps) result is returned (:1302-1304, :1376-1388).
+
+def nufft_forward(obj, x, copy_x=True, grid_only=False, xp=None):
+    """Forward table1 NUFFT (mrrt/nufft/_nufft.py:1275-1397), same signature.
+
+    ``copy_x``: the input is never modified (it is read by the device), so
+    no copy is needed either way.  ``grid_only``: ``x`` is the oversampled
+    grid (prod(Kd), reps); only the table interpolation is applied (no FFT,
+    phase_before, ortho or phase_after; the complex kernel's phase_shift is
+    kept, :1086-1095) and the (M, reps) result is returned (:1302-1304,
+    :1376-1388).  ``xp``: see ``_check_xp``.
     """
+    _check_xp(xp)
     if grid_only:
         t, was_numpy = _to_device(obj, x)
         nk = int(np.prod(obj.Kd))
@@ -606,14 +732,18 @@ def nufft_forward(obj, x, grid_only=False):
     return _to_output(out, was_numpy)
 
 
-def nufft_adj(obj, xk, return_psf=False, grid_only=False):
-    """Adjoint table1 NUFFT (mrrt/nufft/_nufft.py:1459-1578).
+def nufft_adj(obj, xk, copy=True, return_psf=False, grid_only=False, xp=None):
+    """Adjoint table1 NUFFT (mrrt/nufft/_nufft.py:1459-1578), same signature.
 
+    ``copy``: the input is never modified, so no copy is needed either way.
     ``grid_only``: return the raw gridded samples (prod(Kd), reps) before
-    conj(phase_before) and the FFT (:1507-1509).  ``return_psf``: skip
-    conj(phase_after) and return the first repetition's grid with shape Kd
-    (:1495, :1517-1518), e.g. for a Toeplitz point-spread function.
+    conj(phase_before) and the FFT (:1507-1509); it is checked first, as in
+    the reference, so with ``return_psf`` as well the flat grid comes back
+    (without conj(phase_after)).  ``return_psf`` alone: skip conj(phase_after)
+    and return the first repetition's grid with shape Kd (:1495, :1517-1518),
+    e.g. for a Toeplitz point-spread function.  ``xp``: see ``_check_xp``.
     """
+    _check_xp(xp)
     t, was_numpy = _to_device(obj, xk)
     if t.numel() % obj.M != 0:
         raise ValueError("invalid size")
@@ -629,10 +759,10 @@ def nufft_adj(obj, xk, return_psf=False, grid_only=False):
             if ph is not None:
                 y = (y.to(torch.complex128) * torch.conj(ph)[None, :]).to(obj._tcdt)
             g = _raw_table1(obj, False, y.contiguous(), nreps)
-        if return_psf:
-            g0 = g[0].reshape(tuple(reversed(obj.Kd))).permute(*reversed(range(obj.ndim)))
-            return _to_output(g0, was_numpy)
-        return _to_output(g.t(), was_numpy)
+        if grid_only:
+            return _to_output(g.t(), was_numpy)
+        g0 = g[0].reshape(tuple(reversed(obj.Kd))).permute(*reversed(range(obj.ndim)))
+        return _to_output(g0, was_numpy)
     out = obj._adjoint_dev(y, nreps)
     if nreps == 1:
         out = out[..., 0]

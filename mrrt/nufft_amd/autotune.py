@@ -5,9 +5,11 @@ tiled kernels of a plan by
 timing the gather and the scatter stages on the plan's own points with HIP
 events, and caches the choice per geometry in a JSON file
 (``$MI355NUFFT_TUNE_CACHE`` or ``~/.cache/mi355nufft/autotune.json``).
-The cache key holds dimensions, grid, kernel width, table size, precision,
-phasing, number of points (rounded to a power of two), repetitions and
-frames; trajectories of the same kind and size share an entry.
+The cache key holds the library's ABI version, dimensions, image size,
+grid, kernel width, table size, precision, phasing, number of points
+(rounded to a power of two), repetitions, frames and the scatter's
+accumulation; trajectories of the same kind and size share an entry.  An
+entry the library rejects is dropped and tuned again.
 """
 
 import ctypes
@@ -41,8 +43,10 @@ def cache_path():
 
 def _key(op, nreps):
     mb = 1 << int(round(math.log2(max(op.M, 1))))
-    return "|".join(str(v) for v in (op.ndim, op.Kd, op.Jd, op.Ld, op.precision, op.phasing, mb, nreps,
-                                     getattr(op, "nframes", 1)))
+    abi = _lib.get_lib().mi355nufft_abi_version()
+    fxp = int(op.plan_info()["fixed_point_scatter"])
+    return "|".join(str(v) for v in (abi, op.ndim, op.Nd, op.Kd, op.Jd, op.Ld, op.precision, op.phasing, mb, nreps,
+                                     getattr(op, "nframes", 1), fxp))
 
 
 def _load():
@@ -115,9 +119,12 @@ def tune(op, nreps=1, cands=None, reps=3, use_cache=True, verbose=False):
     table = _load() if use_cache else {}
     if use_cache and key in table:
         b, p = table[key]["bin_size"], table[key]["max_points"]
-        op.set_binning(tuple(b), p, "auto", table[key].get("threads", 0))
-        op.set_fft(table[key].get("fft", "auto"))
-        return tuple(b), p, table[key].get("timings", {})
+        try:
+            op.set_binning(tuple(b), p, "auto", table[key].get("threads", 0))
+            op.set_fft(table[key].get("fft", "auto"))
+            return tuple(b), p, table[key].get("timings", {})
+        except (ValueError, KeyError, TypeError):
+            del table[key]  # a stale entry this library rejects: tune again
     cands = cands or candidates(op.ndim)
     nimg = getattr(op, "nframes", 1) * nreps
     nk = int(np.prod(op.Kd))

@@ -23,7 +23,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._nufft import NufftBase, _to_output
+from ._nufft import NufftBase, _to_output, _upload_host
 from .batched import NufftFrames
 
 __all__ = ["SenseOperator"]
@@ -119,8 +119,8 @@ class SenseOperator:
 
     def _in(self, x, shape):
         was_numpy = not isinstance(x, torch.Tensor)
-        if was_numpy:  # pinned staging copy, then an asynchronous transfer
-            t = torch.from_numpy(np.ascontiguousarray(x)).pin_memory().to(self.device, non_blocking=True)
+        if was_numpy:  # through the device's pinned staging buffer
+            t = _upload_host(x, self.device)
         else:
             t = x.to(self.device)
         if tuple(t.shape) != tuple(shape):

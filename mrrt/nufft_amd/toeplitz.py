@@ -22,7 +22,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._nufft import NufftBase, _to_output
+from ._nufft import NufftBase, _to_output, _upload_host
 
 __all__ = ["ToeplitzNormal"]
 
@@ -124,8 +124,8 @@ class ToeplitzNormal:
         """A^H W A x for x of shape Nd or Nd + (nreps,), numpy or tensor;
         numpy in, numpy out."""
         was_numpy = not isinstance(x, torch.Tensor)
-        if was_numpy:  # pinned staging copy, then an asynchronous transfer
-            xt = torch.from_numpy(np.ascontiguousarray(x)).pin_memory().to(self.device, non_blocking=True)
+        if was_numpy:  # through the device's pinned staging buffer
+            xt = _upload_host(x, self.device)
         else:
             xt = x.to(self.device)
         squeeze = tuple(xt.shape) == self.Nd

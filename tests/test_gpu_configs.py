@@ -137,6 +137,24 @@ def test_c3_full_size_adjointness(c3_setup):
     assert e <= 1e-5
 
 
+def test_c3_full_size_adjoint_fixed_point_vs_float64(c3_setup):
+    """C3 adjoint of 2,097,152 random samples: the default 32-bit fixed-point
+    scatter tile against the float64 tile (the one the subset test pins to
+    the oracle).  This is the fixed-point tile's error at full load."""
+    wl, A = c3_setup
+    rng = np.random.default_rng(15)
+    y = (rng.standard_normal(A.M) + 1j * rng.standard_normal(A.M)).astype(np.complex64)
+    assert A.plan_info()["fixed_point_scatter"]
+    xf = A.adj(y)
+    A.set_scatter("float64")
+    try:
+        xd = A.adj(y)
+    finally:
+        A.set_scatter("auto")
+    e = report("c3-c64", "adjoint-fixed-point-vs-float64-tile(full size)", rel_l2(xf, xd))
+    assert e <= 1e-5
+
+
 # ---------------------------------------------------------------------- C4
 @pytest.fixture(scope="module")
 def c4_setup():

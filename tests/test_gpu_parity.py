@@ -531,6 +531,15 @@ def test_grid_only_and_psf_vs_oracle(precision, phasing):
     psf = nufft_adj(A, np.ones(om.shape[0]), return_psf=True)
     assert psf.shape == Kd
     assert rel_l2(psf, B.adj(np.ones(om.shape[0]), return_psf=True)) <= TOL[precision]
+    # both flags: grid_only is checked first (:1507-1509 before :1517), so the
+    # flat gridded samples come back, without conj(phase_after)
+    both = nufft_adj(A, y, return_psf=True, grid_only=True)
+    assert both.shape == (int(np.prod(Kd)), 2)
+    assert rel_l2(both, B.adj(y, return_psf=True, grid_only=True)) <= TOL[precision]
+    # the reference's positional order: (obj, x, copy_x, grid_only, xp) and
+    # (obj, xk, copy, return_psf, grid_only, xp)
+    assert rel_l2(nufft_forward(A, g, True, True, np), B.fft(g, grid_only=True)) <= TOL[precision]
+    assert rel_l2(nufft_adj(A, y, True, False, True, np), B.adj(y, grid_only=True)) <= TOL[precision]
 
 
 def test_two_phase_scatter_is_deterministic():
