@@ -28,10 +28,6 @@ sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
 
-# float64 LDS atomic adds per second over the chip: 64 lanes per 9.06 CU
-# cycles (measured for ds_add_f64 without bank conflicts), 256 CUs, 2.4 GHz
-LDS_F64_ATOMIC_PEAK_LANES_PER_S = 64 / 9.06 * 256 * 2.4e9
-
 KERNEL_NAMES = {"interp": "interp_tiled_kernel<float, 2, false, 6, true>",
                 "spread": "spread_tiled_kernel<float, 2, false, 6, true>"}
 
@@ -53,6 +49,12 @@ def parse():
     ap.add_argument("--graph", action="store_true", help="time a HIP-graph replay instead of eager launches")
     ap.add_argument("--fft", choices=("tuned", "fused", "rocfft"), default="tuned",
                     help="grid FFT: the faster of the two measured here (default), the fused pruned passes, or rocFFT")
+    ap.add_argument("--cpu-only", action="store_true",
+                    help="only the CPU baseline: the reference pipeline (its C kernels + scipy.fft) per stage on the "
+                         "configurations of --cpu-configs, OMP_NUM_THREADS threads; one JSON line per configuration "
+                         "(no GPU needed)")
+    ap.add_argument("--cpu-configs", default="c1,c2,c2_c128,c3,c4,c5")
+    ap.add_argument("--cpu-runs", type=int, default=10, help="--cpu-only: timed runs per configuration (median)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run three 256 MiB device-to-device copies (a known-size reference for "
                          "FETCH_SIZE / WRITE_SIZE counter passes)")
@@ -60,19 +62,15 @@ def parse():
 
 
 def c2_workload(precision="single"):
-    from mrrt.nufft_amd import trajectories as trj
-    Nd, Kd = (256, 256), (512, 512)
-    om = trj.golden_angle_2d(402, 512)
-    x = trj.complex_gaussian(Nd, seed=0) + trj.ellipse_phantom(Nd, 10, seed=0)
-    return dict(Nd=Nd, Kd=Kd, Jd=6, Ld=1024, om=om, x=x, precision=precision)
+    """The 2-D benchmark configuration (mrrt.nufft_amd.workloads.c2)."""
+    from mrrt.nufft_amd import workloads as W
+    return W.c2(precision)
 
 
 def c3_workload():
-    from mrrt.nufft_amd import trajectories as trj
-    Nd, Kd = (128, 128, 128), (256, 256, 256)
-    om = trj.kooshball_3d(8192, 256, seed=0)
-    x = trj.complex_gaussian(Nd, seed=0)
-    return dict(Nd=Nd, Kd=Kd, Jd=6, Ld=512, om=om, x=x, precision="single")
+    """The 3-D benchmark configuration (mrrt.nufft_amd.workloads.c3)."""
+    from mrrt.nufft_amd import workloads as W
+    return W.c3()
 
 
 def interp_bytes(M, ndim, nk, nreps, real_size):
@@ -174,19 +172,49 @@ class GpuRun:
         return out
 
 
+def measure_atomic_peaks():
+    import ctypes
+    import torch
+    from mrrt.nufft_amd import _lib
+    lib = _lib.get_lib()
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    out = {}
+    for kind, name in ((0, "lds_f64"), (1, "lds_u32"), (2, "global_f32")):
+        v = ctypes.c_double(0.0)
+        _lib.check(lib.mi355nufft_measure_atomics(kind, ctypes.byref(v), stream))
+        out[name] = v.value
+    return out
+
+
+def kernel_means(op, fn, n):
+    """(gather_ms, scatter_ms): mean durations of the tiled kernels over n
+    calls of fn, from the plan's dispatch-timestamped HIP events."""
+    op.set_kernel_timing(True)
+    for _ in range(n):
+        fn()
+    _, g = op.read_kernel_timing("interp")
+    _, sc = op.read_kernel_timing("spread")
+    op.set_kernel_timing(False)
+    return g, sc
+
+
 def run_c4(steps=3, autotune=True, binning=None, loop=True):
-    """24 frames x 32 coils, golden-angle frames with 13..89 spokes x 512,
-    Nd 256^2, Kd 512^2, J 6, complex64: one graph-captured frames plan vs a
-    per-frame loop of plain operators (32 coils each).  ``binning`` =
-    (bin_size, max_points, threads) fixes the binning instead of tuning it;
-    ``loop=False`` skips the per-frame comparison (e.g. for profiling)."""
+    """C4: 24 frames x 32 coils, golden-angle frames of 13..89 spokes x 512,
+    Nd 256^2, Kd 512^2, J 6, complex64, with the configuration's complex coil
+    maps: SENSE encoding E x and E^H y of all frames (one image per frame)
+    through one NufftFrames plan, captured in one HIP graph, against a
+    per-frame loop of SenseOperator over plain NufftBase operators.
+    ``binning`` = (bin_size, max_points, threads) fixes the binning instead
+    of tuning it; ``loop=False`` skips the per-frame comparison."""
     import torch
     from mrrt.nufft_amd import NufftBase
-    from mrrt.nufft_amd import trajectories as trj
+    from mrrt.nufft_amd import workloads as W
     from mrrt.nufft_amd.batched import NufftFrames
+    from mrrt.nufft_amd.mri import SenseOperator
     dev = torch.device("cuda", 0)
-    F, C, Nd, Kd = 24, 32, (256, 256), (512, 512)
-    frames = trj.golden_angle_frames(F, 512, 13, 89, seed=0)
+    wl = W.c4()
+    F, C, Nd, Kd = len(wl["frames"]), wl["ncoils"], wl["Nd"], wl["Kd"]
+    frames = wl["frames"]
     B = NufftFrames(Nd, frames, C, Jd=6, Kd=Kd, Ld=1024, device=dev)
     tuned = None
     if binning is not None:
@@ -197,70 +225,69 @@ def run_c4(steps=3, autotune=True, binning=None, loop=True):
         b, pmax, _ = at.tune(B.op, nreps=C, use_cache=False)
         info = B.op.plan_info()
         tuned = {"bin_size": b, "max_points": pmax, "threads": info["threads"]}
-    rng = torch.Generator(device=dev).manual_seed(0)
-    x = torch.randn((F, C) + Nd, dtype=torch.complex64, device=dev, generator=rng)
-    y = torch.empty((C, B.M), dtype=torch.complex64, device=dev)
-    xa = torch.empty_like(x)
-    B.capture(x, y, xa)
+    smaps = torch.from_numpy(wl["smaps"].astype(np.complex64)).to(dev)
+    E = SenseOperator(B, smaps)
+    x = torch.from_numpy(wl["x"].astype(np.complex64)).to(dev)
+    s = torch.cuda.Stream(device=dev)
+    s.wait_stream(torch.cuda.current_stream(dev))
+    with torch.cuda.stream(s):
+        for _ in range(2):  # FFT set-up outside capture; leaves the plan grid zero
+            E.adjoint_dev(E.forward_dev(x))
+    torch.cuda.current_stream(dev).wait_stream(s)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        y_g = E.forward_dev(x)
+        xa_g = E.adjoint_dev(y_g)
+    B.op._sync_stream()
+    t_b = _time_fn(graph.replay, steps, 1)
+    t_eager = _time_fn(lambda: E.adjoint_dev(E.forward_dev(x)), steps, 1)
+    t_l = float("nan")
+    if loop:
+        Es = [SenseOperator(NufftBase(Nd, om, Jd=6, Kd=Kd, Ld=1024, device=dev), smaps) for om in frames]
 
-    def batched():
-        B.replay_forward()
-        B.replay_adjoint()
-
-    ops = [NufftBase(Nd, om, Jd=6, Kd=Kd, Ld=1024, device=dev, order="C") for om in frames] if loop else []
-
-    def loop_frames():
-        for f, op in enumerate(ops):
-            xv = x[f].permute(1, 2, 0)
-            yf = op._forward_dev(xv, C)
-            op._adjoint_dev(yf, C)
-
-    do_loop = loop
-    t_b = _time_fn(batched, steps, 1)
-    t_l = _time_fn(loop_frames, steps, 1) if do_loop else float("nan")
-    # stage split of one batched forward + adjoint (plan stages, 768 images)
-    import ctypes
-    from mrrt.nufft_amd import _lib
-    op, lib, vp = B.op, B.op._lib, ctypes.c_void_p
-    xv = B._image_view(x)
-    xs = _lib.i64arr(list(xv.stride()))
-    g = B._grid
-    nimg = F * C
-    op._sync_stream()
-    stages = {
-        "forward_grid": lambda: _lib.check(lib.mi355nufft_forward_grid(op._plan, vp(xv.data_ptr()), xs,
-                                                                       vp(g.data_ptr()), nimg)),
-        "interp": lambda: _lib.check(lib.mi355nufft_interp(op._plan, vp(g.data_ptr()), vp(y.data_ptr()), C)),
-        "spread": lambda: _lib.check(lib.mi355nufft_spread(op._plan, vp(y.data_ptr()), vp(g.data_ptr()), C)),
-        "adjoint_image": lambda: _lib.check(lib.mi355nufft_adjoint_image(op._plan, vp(g.data_ptr()),
-                                                                         vp(xv.data_ptr()), xs, nimg)),
-    }
-    st = {k: round(_time_fn(fn, steps, 1), 4) for k, fn in stages.items()}
+        def loop_frames():
+            for f, Ef in enumerate(Es):
+                Ef.adjoint_dev(Ef.forward_dev(x[f]))
+        t_l = _time_fn(loop_frames, steps, 1)
+        del Es
+    # kernel durations of the frames transforms (768 coil images)
+    xc = torch.randn((F, C) + Nd, dtype=torch.complex64, device=dev)
+    yc = torch.empty((C, B.M), dtype=torch.complex64, device=dev)
+    xac = torch.empty_like(xc)
+    g_ms, s_ms = kernel_means(B.op, lambda: (B.forward(xc, out=yc), B.adjoint(yc, out=xac)), steps)
     b_bytes = sum(interp_bytes(om.shape[0], 2, 512 * 512, C, 4) for om in frames)
+    del graph
     return {"c4_frames_x_coils": [F, C], "c4_M_total": int(B.M),
-            "c4_batched_graph_ms_per_pair": round(t_b, 3), "c4_per_frame_loop_ms_per_pair": round(t_l, 3),
-            "c4_speedup_batched_vs_loop": round(t_l / t_b, 3),
-            "c4_transforms_per_s_batched": round(2 * F * C * 1000.0 / t_b, 1),
-            "c4_interp_bytes_per_transform": int(b_bytes), "c4_autotuned": tuned, "c4_stages_ms": st,
-            "c4_interp_GBps": round(b_bytes / (st["interp"] * 1e-3) / 1e9, 1),
-            "c4_spread_GBps": round(b_bytes / (st["spread"] * 1e-3) / 1e9, 1)}
+            "c4_sense_graph_ms_per_pair": round(t_b, 3), "c4_sense_eager_ms_per_pair": round(t_eager, 3),
+            "c4_sense_per_frame_loop_ms_per_pair": round(t_l, 3),
+            "c4_speedup_graph_vs_loop": round(t_l / t_b, 3),
+            "c4_transforms_per_s_graph": round(2 * F * C * 1000.0 / t_b, 1),
+            "c4_interp_bytes_per_transform": int(b_bytes), "c4_autotuned": tuned,
+            "c4_gather_kernel_ms": round(g_ms, 4), "c4_scatter_kernel_ms": round(s_ms, 4),
+            "c4_gather_GBps": round(b_bytes / (g_ms * 1e-3) / 1e9, 1),
+            "c4_scatter_GBps": round(b_bytes / (s_ms * 1e-3) / 1e9, 1)}
 
 
 def run_c5(steps=5):
-    """Differentiable spiral: 16 x 4096 samples, 8 coils, Nd 256^2, c64;
-    forward + backward (grad x and grad omega) vs plain forward + adjoint."""
+    """C5: variable-density spiral 16 x 4096, 8 coils, Nd 256^2, c64; loss
+    ||A(x, om) - y||^2 with y = A(x, om_pert) from the perturbed trajectory;
+    forward + backward (grad x and grad omega) vs plain forward + adjoint;
+    and SENSE with 8 coil maps and density weights."""
     import torch
     from mrrt.nufft_amd import NufftBase
     from mrrt.nufft_amd import trajectories as trj
+    from mrrt.nufft_amd import workloads as W
     from mrrt.nufft_amd.autograd import nufft
     dev = torch.device("cuda", 0)
-    Nd, Kd, C = (256, 256), (512, 512), 8
-    om = trj.spiral_2d(16, 4096, seed=0)
+    wl = W.c5()
+    Nd, Kd, C, om = wl["Nd"], wl["Kd"], wl["ncoils"], wl["om"]
     op = NufftBase(Nd, om, Jd=6, Kd=Kd, Ld=1024, device=dev)
-    rng = torch.Generator(device=dev).manual_seed(1)
-    x = torch.randn((C,) + Nd, dtype=torch.complex64, device=dev, generator=rng).requires_grad_(True)
+    x = torch.from_numpy(wl["x"].astype(np.complex64)).to(dev).requires_grad_(True)
     omega = torch.from_numpy(om.astype(np.float32)).to(dev).requires_grad_(True)
-    y_t = torch.randn((C, om.shape[0]), dtype=torch.complex64, device=dev, generator=rng)
+    with torch.no_grad():
+        y_t = nufft(op, x.detach(), torch.from_numpy(wl["om_pert"].astype(np.float32)).to(dev))
+        _ = nufft(op, x.detach(), omega.detach())   # points back to om
 
     def ad_step():
         loss = ((nufft(op, x, omega) - y_t).abs() ** 2).sum()
@@ -281,17 +308,13 @@ def run_c5(steps=5):
     # E x, E^H W y, and the normal operator E^H W E x directly and through
     # the Toeplitz operator
     from mrrt.nufft_amd.mri import SenseOperator
-    g = np.meshgrid(np.linspace(-1, 1, Nd[0]), np.linspace(-1, 1, Nd[1]), indexing="ij")
-    cen = np.random.default_rng(2).uniform(-1, 1, (C, 2))
-    smaps = np.stack([np.exp(-((g[0] - a) ** 2 + (g[1] - b) ** 2)) for a, b in cen]).astype(np.complex64)
+    smaps = trj.coil_maps(C, Nd, seed=3).astype(np.complex64)
     w = np.random.default_rng(3).uniform(0.5, 1.0, om.shape[0]).astype(np.float32)
     E = SenseOperator(op, smaps, weights=w)
     xi = x.detach()[0].contiguous()
-    ys = E.forward_dev(xi)
     t_fa = _time_fn(lambda: E.adjoint_dev(E.forward_dev(xi)), steps, 2)
     E.normal_dev(xi, toeplitz=True)
     t_tn = _time_fn(lambda: E.normal_dev(xi, toeplitz=True), steps, 2)
-    del ys
     out.update({"c5_sense_8coil_fwd_adj_ms": round(t_fa, 4), "c5_sense_8coil_normal_toeplitz_ms": round(t_tn, 4)})
     return out
 
@@ -335,34 +358,193 @@ def _time_fn(fn, steps, warmup):
     return max(s.elapsed_time(e) / steps, wall)
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+class CpuPipeline:
+    """The reference's table1 NUFFT on the host, stage by stage: its own C
+    interpolation kernels (oracle/_ref, built from nufft_table.c with its
+    -ffast-math -fopenmp flags; OpenMP over samples in the forward and over
+    repetitions in the adjoint), the Python glue of nufft_forward / nufft_adj
+    restated (mrrt/nufft/_nufft.py:1275-1397, 1459-1578), and scipy.fft for
+    the grid FFT with as many workers as OpenMP threads (the reference's
+    mrrt.utils.fftn is not in the tree).  Set-up (tables, sn, phases, tm) is
+    excluded, as in the reference's own timings."""
+
+    STAGES_F = ("deapod", "fft", "phase_before", "interp", "phase_after")
+    STAGES_A = ("phase_after_conj", "scatter", "phase_before_conj", "ifft", "crop_scale_sn")
+
+    def __init__(self, Nd, om, Kd, Jd, Ld, precision="single", workers=1):
+        from oracle import nufft_oracle as O
+        from oracle import ref_ctypes as R
+        self.R = R
+        self.kind = "reference" if R.available() else "port"
+        self.B = O.NufftOracle(Nd, om, Jd=Jd, Kd=Kd, Ld=Ld, precision=precision)
+        self.O = O
+        self.workers = workers
+        self.Nd, self.Kd = tuple(Nd), tuple(Kd)
+        self.axes = tuple(range(len(Nd)))
+        self.nk = int(np.prod(Kd))
+
+    def _interp(self, xk):
+        B = self.B
+        if self.kind == "reference":
+            return self.R.ref_interp_forward(xk, B.Kd, B.h, B.Jd, B.Ld, B.tm)
+        return self.O.interp_forward(xk, B.Kd, B.h, B.Jd, B.Ld, B.tm)
+
+    def _spread(self, y):
+        B = self.B
+        if self.kind == "reference":
+            return self.R.ref_interp_adjoint(y, B.Kd, B.h, B.Jd, B.Ld, B.tm)
+        return self.O.interp_adjoint(y, B.Kd, B.h, B.Jd, B.Ld, B.tm)
+
+    def forward(self, x, t):
+        import scipy.fft
+        B = self.B
+        nreps = x.shape[-1]
+        c = time.perf_counter()
+        xk = x * B.sn[..., None]
+        t["deapod"] += time.perf_counter() - c
+        c = time.perf_counter()
+        xk = scipy.fft.fftn(xk, s=self.Kd, axes=self.axes, workers=self.workers)
+        t["fft"] += time.perf_counter() - c
+        c = time.perf_counter()
+        xk = xk.astype(B.cdt, copy=False)
+        xk *= B.phase_before[..., None]
+        xk = xk.reshape((self.nk, nreps), order="F")
+        t["phase_before"] += time.perf_counter() - c
+        c = time.perf_counter()
+        y = self._interp(xk)
+        t["interp"] += time.perf_counter() - c
+        c = time.perf_counter()
+        y *= B.phase_after[:, None]
+        t["phase_after"] += time.perf_counter() - c
+        return y
+
+    def adjoint(self, y, t):
+        import scipy.fft
+        B = self.B
+        nreps = y.shape[-1]
+        c = time.perf_counter()
+        y = y * B.phase_after.conj()[:, None]
+        t["phase_after_conj"] += time.perf_counter() - c
+        c = time.perf_counter()
+        g = self._spread(y)
+        t["scatter"] += time.perf_counter() - c
+        c = time.perf_counter()
+        g = g.reshape(self.Kd + (nreps,), order="F")
+        g *= B.phase_before.conj()[..., None]
+        t["phase_before_conj"] += time.perf_counter() - c
+        c = time.perf_counter()
+        x = scipy.fft.ifftn(g, axes=self.axes, workers=self.workers)
+        t["ifft"] += time.perf_counter() - c
+        c = time.perf_counter()
+        x = x[tuple(slice(n) for n in self.Nd)].astype(B.cdt, copy=False)
+        x *= self.nk * B.adjoint_scalefactor
+        x *= np.conj(B.sn)[..., None]
+        t["crop_scale_sn"] += time.perf_counter() - c
+        return x
+
+    def pair(self, x):
+        """One forward + adjoint; returns the per-stage seconds."""
+        t = dict.fromkeys(self.STAGES_F + self.STAGES_A, 0.0)
+        self.adjoint(self.forward(x, t), t)
+        return t
+
+
+def cpu_configs():
+    """The CPU baseline's workloads: the benchmark configurations, each with
+    its image batch (Fortran Nd + (nreps,))."""
+    from mrrt.nufft_amd import trajectories as trj
+    from mrrt.nufft_amd import workloads as W
+    out = {}
+    w = W.c1()
+    out["c1"] = [(w, w["om"], w["x"].astype(np.complex64)[..., None])]
+    w = W.c2("single")
+    out["c2"] = [(w, w["om"], w["x"].astype(np.complex64)[..., None])]
+    w = W.c2("double")
+    out["c2_c128"] = [(w, w["om"], w["x"].astype(np.complex128)[..., None])]
+    w = W.c3()
+    out["c3"] = [(w, w["om"], w["x"].astype(np.complex64)[..., None])]
+    w = W.c4()
+    C = w["ncoils"]
+    coil = np.moveaxis(w["smaps"][None] * w["x"][:, None], 1, -1).astype(np.complex64)  # (F, *Nd, C)
+    out["c4"] = [(w, om, np.asfortranarray(coil[f])) for f, om in enumerate(w["frames"])]
+    w = W.c5()
+    out["c5"] = [(w, w["om"], np.asfortranarray(np.moveaxis(w["x"], 0, -1).astype(np.complex64)))]
+    return out
+
+
+def cpu_run_config(name, items, runs, warmup=1):
+    """Median over ``runs`` of one forward + adjoint of every item (the
+    per-frame loop for C4); per-stage medians in ms."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", "1"))
+    pipes = [(CpuPipeline(w["Nd"], om, w["Kd"], w["Jd"], w["Ld"], w["precision"], workers=threads), x)
+             for (w, om, x) in items]
+    for _ in range(warmup):
+        for P, x in pipes:
+            P.pair(x)
+    tot, per = [], []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        acc = dict.fromkeys(CpuPipeline.STAGES_F + CpuPipeline.STAGES_A, 0.0)
+        for P, x in pipes:
+            for k, v in P.pair(x).items():
+                acc[k] += v
+        tot.append(time.perf_counter() - t0)
+        per.append(acc)
+    med = float(np.median(tot))
+    stages = {k: round(1e3 * float(np.median([p[k] for p in per])), 3) for k in per[0]}
+    return {"config": name, "threads": threads, "fft_workers": threads, "kind": pipes[0][0].kind,
+            "runs": runs, "ms_per_pair": round(1e3 * med, 3), "pairs_per_s": round(1.0 / med, 4),
+            "stages_ms": stages, "cpu": cpu_model()}
+
+
 def cpu_baseline(seconds):
-    """Reference C kernels (oracle/_ref) + numpy FFT on the host, C2 c64."""
-    from oracle import nufft_oracle as O
-    from oracle import ref_ctypes as R
-    wl = c2_workload("single")
-    kind = "reference" if R.available() else "port"
-    interp = (R.ref_interp_forward, R.ref_interp_adjoint) if R.available() else "numpy"
-    B = O.NufftOracle(wl["Nd"], wl["om"], Jd=6, Kd=wl["Kd"], Ld=1024, precision="single", interp=interp)
-    x = wl["x"].astype(np.complex64)
-    y = B.fft(x)
-    B.adj(y)
+    """The reference pipeline (CpuPipeline) on the GPU box's host cores, C2
+    complex64, for about ``seconds``."""
+    from mrrt.nufft_amd import workloads as W
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    w = W.c2("single")
+    P = CpuPipeline(w["Nd"], w["om"], w["Kd"], w["Jd"], w["Ld"], "single", workers=threads)
+    x = w["x"].astype(np.complex64)[..., None]
+    P.pair(x)
     t0 = time.perf_counter()
     n = 0
     while True:
-        y = B.fft(x)
-        B.adj(y)
+        P.pair(x)
         n += 1
         if time.perf_counter() - t0 > seconds:
             break
     dt = (time.perf_counter() - t0) / n
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": round(1.0 / dt, 4), "unit": "pairs/s", "cores": threads, "kind": kind,
-            "sample": "%d forward+adjoint pairs of the same 2-D 256^2 workload (complex64); reference C "
-                      "table1 kernels (OpenMP forward, 1-thread-per-coil adjoint) + numpy FFT" % n}
+    return {"value": round(1.0 / dt, 4), "unit": "pairs/s", "cores": threads, "kind": P.kind,
+            "sample": "%d forward+adjoint pairs of the 2-D 256^2 / 512^2 golden-angle workload (complex64) on the "
+                      "GPU box's host (%s, %d threads): the reference's C table1 kernels (OpenMP over samples "
+                      "forward, one thread per coil adjoint) + scipy.fft with %d workers" % (n, cpu_model(), threads,
+                                                                                              threads)}
+
+
+def main_cpu(args):
+    cfgs = cpu_configs()
+    for name in args.cpu_configs.split(","):
+        print("bench: cpu %s (%d runs, %s threads)" % (name, args.cpu_runs, os.environ.get("OMP_NUM_THREADS", "1")),
+              file=sys.stderr, flush=True)
+        print(json.dumps(cpu_run_config(name, cfgs[name], args.cpu_runs)), flush=True)
 
 
 def main():
     args = parse()
+    if args.cpu_only:
+        main_cpu(args)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: every run here uses a single MI355X "
               "(the path shards by frames and coils, or by slab; only one GPU is measured)", file=sys.stderr)
@@ -420,18 +602,24 @@ def main():
                 "bytes_per_launch": b,
                 "other_kernel": {"kernel": KERNEL_NAMES["interp" if dom == "spread" else "spread"],
                                  "kernel_ms": round(kern["interp" if dom == "spread" else "spread"][1], 5)}}
-    # the scatter's LDS-atomic view: M * J^d complex accumulations per launch,
-    # two float64 LDS atomic adds each, against the LDS atomic peak
+    # the scatter's atomic roofs, measured on this GPU: LDS float64 and
+    # 32-bit integer atomic adds at scattered addresses, and coalesced global
+    # float32 atomic adds (mi355nufft_measure_atomics)
+    peaks = measure_atomic_peaks()
+    print("bench: atomic peaks (lane ops/s) %s" % json.dumps({k: "%.3g" % v for k, v in peaks.items()}),
+          file=sys.stderr, flush=True)
+    # the C2 scatter accumulates in float64: M * J^d complex accumulations
+    # per launch, two ds_add_f64 lane operations each
     acc = run.M * 6 ** 2
     t_sc = kern["spread"][1] * 1e-3
-    lane_peak = LDS_F64_ATOMIC_PEAK_LANES_PER_S
     roofline["scatter_lds_atomics"] = {
         "accumulations_per_s": round(acc / t_sc, 1),
         "lane_atomics_per_s": round(2 * acc / t_sc, 1),
-        "peak_lane_atomics_per_s": round(lane_peak, 1),
-        "frac": round(2 * acc / t_sc / lane_peak, 4),
-        "peak_basis": "ds_add_f64: 64 lanes per 9.06 CU-cycles (measured, conflict-free), 256 CUs, 2.4 GHz",
+        "peak_lane_atomics_per_s": round(peaks["lds_f64"], 1),
+        "frac": round(2 * acc / t_sc / peaks["lds_f64"], 4),
+        "peak_basis": "ds_add_f64 at scattered addresses, measured here (mi355nufft_measure_atomics kind 0)",
     }
+    roofline["atomic_peaks_lane_ops_per_s"] = {k: round(v, 1) for k, v in peaks.items()}
     extra = {}
     if not args.no_extra:
         other = "rocfft" if fft_choice == "auto" else "auto"
@@ -441,14 +629,16 @@ def main():
         extra["c2_pairs_per_s_with_fft_" + ("fused" if other == "auto" else "rocfft")] = round(1000.0 / t_other, 2)
         # the Python drop-in on host (numpy) arrays: PCIe copies included
         xh = np.asfortranarray(wl["x"].astype(np.complex64))
-        yh = run.A.fft(xh)
-        run.A.adj(yh)
-        nh = max(5, args.steps // 5)
+        for _ in range(5):
+            yh = run.A.fft(xh)
+            run.A.adj(yh)
+        nh = max(50, args.steps)
         t0 = time.perf_counter()
         for _ in range(nh):
             yh = run.A.fft(xh)
             run.A.adj(yh)
         extra["c2_host_arrays_pairs_per_s"] = round(nh / (time.perf_counter() - t0), 2)
+        extra["c2_host_arrays_pairs_timed"] = nh
         # Toeplitz normal operator A^H A on the same points (circulant 512^2)
         from mrrt.nufft_amd.toeplitz import ToeplitzNormal
         tn = ToeplitzNormal(run.A)
@@ -468,11 +658,19 @@ def main():
         r3 = GpuRun(c3_workload())
         t3 = r3.time(r3.pair, 5, 1)
         st3 = r3.stage_times(5, 1)
+        g3, s3 = kernel_means(r3.A, r3.pair, 5)
         extra["c3_3d_complex64_pairs_per_s"] = round(1000.0 / t3, 3)
         extra["c3_stages_ms"] = {k: round(v, 4) for k, v in st3.items()}
         b3 = interp_bytes(r3.M, 3, r3.nk, 1, 4)
-        extra["c3_interp_GBps"] = round(b3 / (st3["interp"] * 1e-3) / 1e9, 1)
-        extra["c3_spread_GBps"] = round(b3 / (st3["spread"] * 1e-3) / 1e9, 1)
+        extra["c3_gather_kernel_ms"] = round(g3, 4)
+        extra["c3_scatter_kernel_ms"] = round(s3, 4)
+        extra["c3_gather_GBps"] = round(b3 / (g3 * 1e-3) / 1e9, 1)
+        extra["c3_scatter_GBps"] = round(b3 / (s3 * 1e-3) / 1e9, 1)
+        info3 = r3.A.plan_info()
+        acc3 = r3.M * 6 ** 3
+        lds_peak = peaks["lds_u32"] if info3["fixed_point_scatter"] else peaks["lds_f64"]
+        extra["c3_scatter_lds_atomic_frac"] = round(2 * acc3 / (s3 * 1e-3) / lds_peak, 4)
+        extra["c3_scatter_accumulation"] = "fixed32" if info3["fixed_point_scatter"] else "float64"
         del r3
         torch.cuda.empty_cache()
         extra.update(run_sparse(c3_workload(), "c3", 5, 1, "auto"))

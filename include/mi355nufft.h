@@ -334,6 +334,20 @@ int mi355nufft_clear_sparse(mi355nufft_plan *plan);
 int mi355nufft_sparse_apply(mi355nufft_plan *plan, int adjoint, const void *in,
                             void *out, int32_t nreps);
 
+/*
+ * Atomic throughput of this GPU, measured by timing one kernel launch
+ * (measurement only; no counterpart in the reference).  The rates the
+ * scatter is compared with:
+ *   kind 0  LDS float64 atomic adds (ds_add_f64) at scattered addresses
+ *   kind 1  LDS 32-bit integer atomic adds (ds_add_u32) at scattered
+ *           addresses
+ *   kind 2  global float32 atomic adds without return, coalesced, over
+ *           256 MiB
+ * ops_per_s receives lane operations per second over the whole device.
+ * Allocates and frees its own buffer; waits for the stream.
+ */
+int mi355nufft_measure_atomics(int kind, double *ops_per_s, void *stream);
+
 /* ---- multi-coil encoding (no plan) ---- */
 
 /*

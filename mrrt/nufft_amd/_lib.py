@@ -61,6 +61,7 @@ _SIGNATURES = {
     "mi355nufft_adjoint_image": (c_int, [c_vp, c_vp, c_vp, c_i64p, c_i32]),
     "mi355nufft_set_fft": (c_int, [c_vp, c_int]),
     "mi355nufft_set_scatter": (c_int, [c_vp, c_int]),
+    "mi355nufft_measure_atomics": (c_int, [c_int, c_dblp, c_vp]),
     "mi355nufft_set_sparse": (c_int, [c_vp, c_vp, c_vp, c_vp]),
     "mi355nufft_clear_sparse": (c_int, [c_vp]),
     "mi355nufft_coil_expand": (c_int, [c_int, c_vp, c_vp, c_i64, c_i32, c_i32, c_vp, c_vp]),

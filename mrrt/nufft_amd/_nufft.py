@@ -14,7 +14,6 @@ reference's CPU path) or torch tensors (results stay on the device).
 """
 
 import ctypes
-import threading
 import warnings
 from math import sqrt
 
@@ -542,75 +541,33 @@ def _numel(x):
     return x.numel() if isinstance(x, torch.Tensor) else np.size(x)
 
 
-class _HostStaging(object):
-    """Pinned host buffers of one device for numpy inputs and outputs,
-    allocated once and grown on demand (a fresh pinned allocation per call
-    costs more than the transfer).  An event recorded after each upload
-    guards the input buffer against being overwritten while the transfer is
-    still in flight; the output buffer is only used under a stream
-    synchronisation.  Results are copied into ordinary numpy arrays, so
-    callers never hold page-locked memory."""
-
-    def __init__(self):
-        self.lock = threading.Lock()
-        self.inbuf = None
-        self.outbuf = None
-        self.in_event = None
-
-    @staticmethod
-    def _grow(buf, nbytes):
-        if buf is None or buf.numel() < nbytes:
-            buf = torch.empty(max(nbytes, 1 << 16), dtype=torch.uint8, pin_memory=True)
-        return buf
-
-    def upload(self, x, device):
-        """numpy x -> device tensor of the same shape and dtype (asynchronous)."""
-        tdt = _TORCH_DTYPE[x.dtype]
-        with self.lock:
-            if self.in_event is not None:
-                self.in_event.synchronize()
-            self.inbuf = self._grow(self.inbuf, x.nbytes)
-            staged = self.inbuf[:x.nbytes].view(tdt).reshape(x.shape)
-            np.copyto(staged.numpy(), x, casting="no")
-            t = staged.to(device, non_blocking=True)
-            if self.in_event is None:
-                self.in_event = torch.cuda.Event()
-            self.in_event.record(torch.cuda.current_stream(device))
-        return t
-
-    def download(self, t):
-        """device tensor -> new ordinary numpy array (waits for the copy)."""
-        nbytes = t.numel() * t.element_size()
-        with self.lock:
-            self.outbuf = self._grow(self.outbuf, nbytes)
-            staged = self.outbuf[:nbytes].view(t.dtype).reshape(t.shape)
-            staged.copy_(t, non_blocking=True)
-            torch.cuda.current_stream(t.device).synchronize()
-            return staged.numpy().copy()
-
-
 _TORCH_DTYPE = {np.dtype(np.complex64): torch.complex64, np.dtype(np.complex128): torch.complex128,
                 np.dtype(np.float32): torch.float32, np.dtype(np.float64): torch.float64}
-_STAGING = {}
-_STAGING_LOCK = threading.Lock()
-
-
-def _staging(device):
-    key = (device.type, device.index if device.index is not None else torch.cuda.current_device())
-    with _STAGING_LOCK:
-        st = _STAGING.get(key)
-        if st is None:
-            st = _STAGING[key] = _HostStaging()
-    return st
 
 
 def _upload_host(x, device):
-    """numpy array -> device tensor of the same shape (floating or complex
-    dtypes; others as float64), through the device's pinned staging buffer."""
+    """numpy array -> device tensor of the same shape, dtype and memory order
+    (floating or complex dtypes; others as float64), asynchronous.
+
+    The array is copied into page-locked memory from torch's caching host
+    allocator (a cached block, not a new pinned allocation per call), as one
+    flat memcpy in the array's own order (C or Fortran), and that flat
+    buffer goes to the device in one DMA; the device tensor then takes the
+    array's strides.  The allocator keeps the block until the copy has
+    completed."""
     x = np.asarray(x)
     if x.dtype not in _TORCH_DTYPE:
         x = x.astype(np.float64)
-    return _staging(device).upload(x, device)
+    if not (x.flags.c_contiguous or x.flags.f_contiguous):
+        x = np.ascontiguousarray(x)
+    tdt = _TORCH_DTYPE[x.dtype]
+    pinned = torch.empty(max(x.size, 1), dtype=tdt, pin_memory=True)[:x.size]
+    order = "C" if x.flags.c_contiguous else "F"
+    np.copyto(pinned.numpy(), x.reshape(-1, order=order), casting="no")
+    flat = pinned.to(device, non_blocking=True)
+    if order == "C":
+        return flat.reshape(x.shape)
+    return flat.reshape(x.shape[::-1]).permute(*reversed(range(x.ndim)))
 
 
 def _to_device(obj, x):
@@ -636,11 +593,21 @@ def _to_device(obj, x):
 
 
 def _to_output(t, was_numpy):
-    """Device results for numpy callers: copied through the device's pinned
-    staging buffer into a new ordinary numpy array (synchronous)."""
-    if was_numpy:
-        return _staging(t.device).download(t)
-    return t
+    """Device results for numpy callers: a new numpy array (synchronous).
+
+    Dense results (any dimension order) land in page-locked memory from
+    torch's caching host allocator with the device tensor's strides, so the
+    copy is one DMA at pinned rate; the array keeps that block until it is
+    freed, and the allocator then reuses it.  Other layouts take an ordinary
+    copy."""
+    if not was_numpy:
+        return t
+    if t.is_contiguous() or t.permute(*reversed(range(t.dim()))).is_contiguous():
+        out = torch.empty_strided(t.shape, t.stride(), dtype=t.dtype, pin_memory=True)
+        out.copy_(t, non_blocking=True)
+        torch.cuda.current_stream(t.device).synchronize()
+        return out.numpy()
+    return t.cpu().numpy()
 
 
 def _raw_table1(obj, fwd, data, nreps):

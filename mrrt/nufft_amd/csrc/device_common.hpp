@@ -60,6 +60,9 @@ template <typename T>
 __device__ __forceinline__ long long first_tap(T t, int J) {
     double kd = floor((double)t - 0.5 * (double)J);
     kd = fmin(fmax(kd, -1.0e9), 1.0e9);  // NaN -> -1e9, keeps the cast defined
+    // |k + j| < 2^31 for every tap j, so callers convert tap positions
+    // through int: the same value as from 64 bits, in one instruction
+    // instead of a software int64 -> float conversion
     return 1 + (long long)kd;
 }
 
@@ -95,7 +98,7 @@ __device__ __forceinline__ int axis_coefs(T t, int L, const AxisGeom& g,
 #pragma unroll
     for (int j = 0; j < JMAX; ++j) {
         if (FIXED || j < g.J) {
-            const T p = (t - (T)(k + j)) * tL;
+            const T p = (t - (T)(int)(k + j)) * tL;
             const T pc = fmin_t(fmax_t(p, (T)g.nlo), (T)g.nhi);
             const int n = (int)floor_t(pc);
             const T alf = p - (T)n;
@@ -119,7 +122,7 @@ __device__ __forceinline__ int axis_coefs(T t, int L, const AxisGeom& g,
 #pragma unroll
     for (int j = 0; j < JMAX; ++j) {
         if (FIXED || j < g.J) {
-            const T p = (t - (T)(k + j)) * tL;
+            const T p = (t - (T)(int)(k + j)) * tL;
             const T pc = fmin_t(fmax_t(p, (T)g.nlo), (T)g.nhi);
             const int n = (int)floor_t(pc);
             const T alf = p - (T)n;
@@ -148,7 +151,7 @@ __device__ __forceinline__ int axis_coefs_d(T t, int L, const AxisGeom& g, const
 #pragma unroll
     for (int j = 0; j < JMAX; ++j) {
         if (FIXED || j < g.J) {
-            const T p = (t - (T)(k + j)) * tL;
+            const T p = (t - (T)(int)(k + j)) * tL;
             const T pc = fmin_t(fmax_t(p, (T)g.nlo), (T)g.nhi);
             const int n = (int)floor_t(pc);
             const T alf = p - (T)n;
@@ -172,7 +175,7 @@ __device__ __forceinline__ int axis_coefs_d(T t, int L, const AxisGeom& g, const
 #pragma unroll
     for (int j = 0; j < JMAX; ++j) {
         if (FIXED || j < g.J) {
-            const T p = (t - (T)(k + j)) * tL;
+            const T p = (t - (T)(int)(k + j)) * tL;
             const T pc = fmin_t(fmax_t(p, (T)g.nlo), (T)g.nhi);
             const int n = (int)floor_t(pc);
             const T alf = p - (T)n;

@@ -144,6 +144,12 @@ struct mi355nufft_plan {
     // prod_d max |h_d| over the plan's tables (bound of every coefficient
     // product); 0 when unknown (raw entry points: caller tables)
     double tab_cmax = 0.0;
+    // 1: the gather's tap sums keep the reference's roundings (raw entry
+    // points); 0: fused multiply-adds for the J = 6 kernels
+    int exact_taps = 0;
+    // 1 while a sparse matrix is set: its rows follow the bin order, and
+    // the default bins are then the float64-tile ones
+    int sparse_bins = 0;
     int bin_req[3] = {0, 0, 0};  // requested bin size per axis (0: default)
     int P_req = 0;               // requested points per subproblem (0: default)
     int threads_req = 0;         // requested threads per tiled workgroup (0: default)
@@ -187,9 +193,13 @@ namespace mi355host {
 // Kd 256^3 for a relative error of about 3e-6 (2-D gained under 10% for
 // 7x the error), or on request; it needs the plan's table bound.  Else the
 // tile accumulates in float64.
-inline bool use_fxp(const mi355nufft_plan* p) {
+inline bool fxp_eligible(const mi355nufft_plan* p) {
     if (p->precision != 0 || !(p->tab_cmax > 0.0 && p->tab_cmax < 1e30) || p->scatter_acc == 1) return false;
     return p->scatter_acc == 2 || p->ndim == 3;
+}
+inline bool use_fxp(const mi355nufft_plan* p) {
+    // the int32 tile's bound holds up to 2^20 points per subproblem
+    return fxp_eligible(p) && p->bg.P <= (1 << 20);
 }
 inline size_t tile_bytes(const mi355nufft_plan* p) {
     return (size_t)p->bg.tile * 2 * (use_fxp(p) ? sizeof(int) : sizeof(double));

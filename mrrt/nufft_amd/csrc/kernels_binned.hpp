@@ -243,6 +243,7 @@ struct TiledArgs {
     int rc;                 // gather: repetitions staged together (LDS holds rc tiles)
     int own_ok;             // scatter: every K_d is a multiple of B_d (see spread_tiled_kernel)
     float cmax;             // scatter: > 0 accumulates in fixed point; bound of |coefficient product|
+    int exact;              // gather: 1 keeps the reference's roundings (no fused multiply-add)
 };
 
 // Copy tm and the per-sample phase into sorted (bin) order, so the tiled
@@ -400,7 +401,8 @@ __device__ __forceinline__ void point_taps(const TiledArgs<T, CK>& a, const Axis
 }
 
 // sum over the J^d taps of the staged tile, in the reference's order
-template <typename T, int NDIM, bool CK, int JMAX, bool FIXED>
+// (FUSED: with fused multiply-adds)
+template <typename T, int NDIM, bool CK, int JMAX, bool FIXED, bool FUSED = false>
 __device__ __forceinline__ cx<T> gather_taps(const cx<T>* tile, const PointTaps<T, CK, JMAX>& pt,
                                              const AxisGeom* ax, int T0, int T1, bool has_ph) {
     T sr = 0, si = 0;
@@ -409,7 +411,7 @@ __device__ __forceinline__ cx<T> gather_taps(const cx<T>* tile, const PointTaps<
         for (int j1 = 0; j1 < JMAX; ++j1)
             if (FIXED || j1 < ax[0].J) {
                 const cx<T> v = tile[pt.l[0] + j1];
-                madd<T, CK>(sr, si, pt.cr[0][j1], pt.ci[0][j1], v.x, v.y);
+                madd_sel<T, CK, FUSED>(sr, si, pt.cr[0][j1], pt.ci[0][j1], v.x, v.y);
             }
     } else if constexpr (NDIM == 2) {
 #pragma unroll
@@ -421,14 +423,50 @@ __device__ __forceinline__ cx<T> gather_taps(const cx<T>* tile, const PointTaps<
                 for (int j1 = 0; j1 < JMAX; ++j1)
                     if (FIXED || j1 < ax[0].J) {
                         const cx<T> v = row[j1];
-                        madd<T, CK>(s1r, s1i, pt.cr[0][j1], pt.ci[0][j1], v.x, v.y);
+                        madd_sel<T, CK, FUSED>(s1r, s1i, pt.cr[0][j1], pt.ci[0][j1], v.x, v.y);
                     }
-                madd<T, CK>(sr, si, pt.cr[1][j2], pt.ci[1][j2], s1r, s1i);
+                madd_sel<T, CK, FUSED>(sr, si, pt.cr[1][j2], pt.ci[1][j2], s1r, s1i);
             }
+    } else if constexpr (FIXED) {
+        // The 36 rows of the (j2, j3) plane in the same order, each loaded
+        // R rows ahead of its use: left alone, the compiler keeps two LDS
+        // reads in flight and waits on each pair (measured 5% faster at
+        // Kd 256^3 with R = 4, at the same occupancy).
+        constexpr int J = JMAX, R = 4, NR = JMAX * JMAX;
+        cx<T> buf[R][J];
+        auto rowp = [&](int r) { return tile + ((pt.l[2] + r / J) * T1 + (pt.l[1] + r % J)) * T0 + pt.l[0]; };
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const cx<T>* q = rowp(r);
+#pragma unroll
+            for (int j1 = 0; j1 < J; ++j1) buf[r][j1] = q[j1];
+        }
+        T s2r = 0, s2i = 0;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            cx<T> cur[J];
+#pragma unroll
+            for (int j1 = 0; j1 < J; ++j1) cur[j1] = buf[r % R][j1];
+            if (r + R < NR) {
+                const cx<T>* q = rowp(r + R);
+#pragma unroll
+                for (int j1 = 0; j1 < J; ++j1) buf[r % R][j1] = q[j1];
+            }
+            T s1r = 0, s1i = 0;
+#pragma unroll
+            for (int j1 = 0; j1 < J; ++j1)
+                madd_sel<T, CK, FUSED>(s1r, s1i, pt.cr[0][j1], pt.ci[0][j1], cur[j1].x, cur[j1].y);
+            madd_sel<T, CK, FUSED>(s2r, s2i, pt.cr[1][r % J], pt.ci[1][r % J], s1r, s1i);
+            if (r % J == J - 1) {
+                madd_sel<T, CK, FUSED>(sr, si, pt.cr[2][r / J], pt.ci[2][r / J], s2r, s2i);
+                s2r = 0;
+                s2i = 0;
+            }
+        }
     } else {
-#pragma unroll FIXED ? JMAX : 1
+#pragma unroll 1
         for (int j3 = 0; j3 < JMAX; ++j3)
-            if (FIXED || j3 < ax[2].J) {
+            if (j3 < ax[2].J) {
                 T s2r = 0, s2i = 0;
 #pragma unroll
                 for (int j2 = 0; j2 < JMAX; ++j2)
@@ -439,11 +477,11 @@ __device__ __forceinline__ cx<T> gather_taps(const cx<T>* tile, const PointTaps<
                         for (int j1 = 0; j1 < JMAX; ++j1)
                             if (FIXED || j1 < ax[0].J) {
                                 const cx<T> v = row[j1];
-                                madd<T, CK>(s1r, s1i, pt.cr[0][j1], pt.ci[0][j1], v.x, v.y);
+                                madd_sel<T, CK, FUSED>(s1r, s1i, pt.cr[0][j1], pt.ci[0][j1], v.x, v.y);
                             }
-                        madd<T, CK>(s2r, s2i, pt.cr[1][j2], pt.ci[1][j2], s1r, s1i);
+                        madd_sel<T, CK, FUSED>(s2r, s2i, pt.cr[1][j2], pt.ci[1][j2], s1r, s1i);
                     }
-                madd<T, CK>(sr, si, pt.cr[2][j3], pt.ci[2][j3], s2r, s2i);
+                madd_sel<T, CK, FUSED>(sr, si, pt.cr[2][j3], pt.ci[2][j3], s2r, s2i);
             }
     }
     if (!has_ph) return mkcx<T>(sr, si);
@@ -597,7 +635,8 @@ __device__ __forceinline__ float block_sum(float v, float* red) {
 
 // MULTI: repetitions (coils) are processed a.rc at a time: a.rc tiles are
 // staged, and each point's coefficients are computed once for all of them.
-template <typename T, int NDIM, bool CK, int JMAX, bool FIXED, bool MULTI>
+// FUSED: the tap sums use fused multiply-adds.
+template <typename T, int NDIM, bool CK, int JMAX, bool FIXED, bool MULTI, bool FUSED = false>
 __global__ __launch_bounds__(256) void interp_tiled_kernel(TiledArgs<T, CK> a) {
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* tiles = reinterpret_cast<cx<T>*>(smem);
@@ -617,7 +656,7 @@ __global__ __launch_bounds__(256) void interp_tiled_kernel(TiledArgs<T, CK> a) {
                 for (int i = s.start + tid; i < s.end; i += blockDim.x) {
                     PointTaps<T, CK, JMAX> pt;
                     point_taps<T, NDIM, CK, JMAX, FIXED>(a, ax, s, i, pt);
-                    ia.out[(int64_t)r * ia.M + pt.m] = gather_taps<T, NDIM, CK, JMAX, FIXED>(tiles, pt, ax, T0, T1, a.phs);
+                    ia.out[(int64_t)r * ia.M + pt.m] = gather_taps<T, NDIM, CK, JMAX, FIXED, FUSED>(tiles, pt, ax, T0, T1, a.phs);
                 }
                 __syncthreads();
             }
@@ -634,7 +673,7 @@ __global__ __launch_bounds__(256) void interp_tiled_kernel(TiledArgs<T, CK> a) {
                 point_taps<T, NDIM, CK, JMAX, FIXED>(a, ax, s, i, pt);
                 for (int rr = 0; rr < rc; ++rr)
                     ia.out[(int64_t)(r0 + rr) * ia.M + pt.m] =
-                        gather_taps<T, NDIM, CK, JMAX, FIXED>(tiles + (int64_t)rr * bg.tile, pt, ax, T0, T1, a.phs);
+                        gather_taps<T, NDIM, CK, JMAX, FIXED, FUSED>(tiles + (int64_t)rr * bg.tile, pt, ax, T0, T1, a.phs);
             }
             __syncthreads();
         }
@@ -656,14 +695,17 @@ __device__ __forceinline__ cx<T> sample_times_conj_phase(const InterpArgs<T, CK>
 // gfx950 an LDS float64 atomic add at scattered addresses costs about 21
 // CU-cycles per wave64 instruction and a 32-bit integer one about 7 (a
 // float32 one about 194), and the LDS atomics are most of the scatter's time.
-// Per subproblem and repetition the taps are scaled by 2^sh, sh = 30 - e with
-// 2^(e-1) <= A * cmax < 2^e, A = sum over its samples of |Re y| + |Im y| and
-// cmax bounding every coefficient product: every tap and every cell sum stays
-// below 2^30 + P in magnitude (no overflow), and each tap is rounded to a
-// step of at most 2^-29 A cmax.  The scale is a power of two, so scaling and
-// unscaling are exact, and the integer sums do not depend on the order of
-// the atomics.  Non-finite samples make the whole tile NaN.  Without FXP
-// (and for complex128) the tile accumulates in float64.
+// Per subproblem and repetition the taps are scaled by 2^sh, sh = 31 - e with
+// 2^(e-1) <= 1.001 A cmax < 2^e, A = sum over its samples of |Re y| + |Im y|
+// and cmax bounding every coefficient product.  Each tap component is at
+// most |Re y| + |Im y| times cmax, so every cell sum stays below
+// 2^31 / 1.001 plus half a unit per point for the rounding: within int32
+// for up to 2^20 points per subproblem (the integer adds wrap modulo 2^32,
+// so only the final sums must fit).  Each tap is rounded to a step of at most
+// 2^-30 1.001 A cmax.  The scale is a power of two, so scaling and unscaling
+// are exact, and the integer sums do not depend on the order of the atomics.
+// Non-finite samples make the whole tile NaN.  Without FXP (and for
+// complex128) the tile accumulates in float64.
 template <typename T, int NDIM, bool CK, int JMAX, bool FIXED, bool FXP>
 __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
     static_assert(!FXP || sizeof(T) == 4, "fixed-point accumulation is for complex64");
@@ -695,11 +737,11 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
                     const cx<T> y = ia.in[(int64_t)r * ia.M + m];
                     part += fabsf(y.x) + fabsf(y.y);  // |f| = |y| per component bound (|phase| = 1)
                 }
-                const float A = block_sum(part, red) * a.cmax * 1.0001f;
+                const float A = block_sum(part, red) * a.cmax * 1.001f;
                 bad = !(A <= 3.0e38f);
                 int e = 0;
                 (void)frexpf(A, &e);
-                sh = 30 - e;
+                sh = 31 - e;
             }
             __syncthreads();
             for (int i = s.start + tid; i < s.end; i += blockDim.x) {

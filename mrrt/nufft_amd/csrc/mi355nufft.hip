@@ -9,6 +9,7 @@
 
 #include "internal.hpp"
 #include "kernels_coil.hpp"
+#include "kernels_probe.hpp"
 
 namespace mi355host {
 
@@ -708,6 +709,14 @@ void default_bins(const mi355nufft_plan* p, int B[3], int& P) {
         B[0] = 16;
         B[1] = 16;
         P = 256;
+    } else if (fxp_eligible(p) && !p->sparse_bins) {
+        // the fixed-point scatter tile of 21^3 cells takes 74 KB of LDS:
+        // bigger bins cut the halo and the flush (measured at Kd 256^3:
+        // scatter 0.59 -> 0.45 ms, gather unchanged)
+        B[0] = 16;
+        B[1] = 16;
+        B[2] = 16;
+        P = 2048;
     } else {
         B[0] = 16;
         B[1] = 8;
@@ -1099,6 +1108,7 @@ void raw_plan(mi355nufft_plan& q, int ndim, const int64_t* Kd, const int32_t* Jd
     q.has_phase = 0;
     q.nframes = 1;
     q.lds_limit = lds;
+    q.exact_taps = 1;
     setup_bin_geom(&q);
 }
 
@@ -1624,6 +1634,22 @@ int mi355nufft_adjoint_image(mi355nufft_plan* p, void* grid, void* x, const int6
                         : adjoint_image_impl<float>(p, grid, x, xs, nimg, 0);
 }
 
+// Sparse-matrix rows follow the bin order of the sort made before they are
+// built: with a matrix set, the default 3-D complex64 bins are the 16x8x8
+// ones (measured faster for the row product at Kd 256^3 than the 16^3 bins
+// of the fixed-point scatter tile).  Re-bins the points when that changes
+// the geometry.
+static int set_sparse_bins(mi355nufft_plan* p, int on) {
+    if (p->sparse_bins == on) return MI355NUFFT_OK;
+    int B0[3], P0, B1[3], P1;
+    default_bins(p, B0, P0);
+    p->sparse_bins = on;
+    default_bins(p, B1, P1);
+    const bool same = P0 == P1 && B0[0] == B1[0] && B0[1] == B1[1] && B0[2] == B1[2];
+    if (same || !(p->d_tm && p->M > 0)) return MI355NUFFT_OK;
+    return bin_points(p);
+}
+
 int mi355nufft_set_sparse(mi355nufft_plan* p, const void* const* coef, const int32_t* const* kidx,
                           const void* phase) {
     g_err.clear();
@@ -1636,6 +1662,7 @@ int mi355nufft_set_sparse(mi355nufft_plan* p, const void* const* coef, const int
             if (!coef[d] || !kidx[d]) return fail(MI355NUFFT_EINVAL, "coef[%d] and kidx[%d] are required", d, d);
     }
     if (phase && p->phasing == 0) return fail(MI355NUFFT_EINVAL, "phase is for the complex kernel (phasing 1)");
+    RET_IF(set_sparse_bins(p, 1));
     if (p->precision)
         return p->phasing ? sparse_build<double, true>(p, coef, kidx, phase)
                           : sparse_build<double, false>(p, coef, kidx, phase);
@@ -1646,7 +1673,7 @@ int mi355nufft_clear_sparse(mi355nufft_plan* p) {
     g_err.clear();
     RET_IF(check_plan(p));
     sparse_free(p);
-    return MI355NUFFT_OK;
+    return set_sparse_bins(p, 0);
 }
 
 int mi355nufft_sparse_apply(mi355nufft_plan* p, int adjoint, const void* in, void* out, int32_t nreps) {
@@ -1727,6 +1754,59 @@ int mi355nufft_set_scatter(mi355nufft_plan* p, int accumulate) {
     p->scatter_acc = accumulate;
     // the tile size changes, and with it whether the tiled kernels fit
     if (p->d_tm && p->M > 0) return bin_points(p);
+    return MI355NUFFT_OK;
+}
+
+int mi355nufft_measure_atomics(int kind, double* ops_per_s, void* stream) {
+    g_err.clear();
+    if (!ops_per_s) return fail(MI355NUFFT_EINVAL, "ops_per_s is null");
+    if (kind < 0 || kind > 2) return fail(MI355NUFFT_EINVAL, "kind must be 0, 1 or 2");
+    hipStream_t s = (hipStream_t)stream;
+    int dev = 0, ncu = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const unsigned nblk = (unsigned)(ncu > 0 ? 8 * ncu : 2048);
+    const int iters = 2048;
+    const long long gn = 64ll << 20;  // 256 MiB of float32
+    const int greps = 4;
+    const size_t bytes = kind == 2 ? (size_t)gn * sizeof(float) : (size_t)nblk * 32768;
+    void* buf = nullptr;
+    if (hipMalloc(&buf, bytes) != hipSuccess) return fail(MI355NUFFT_ENOMEM, "probe buffer of %zu bytes", bytes);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = MI355NUFFT_OK;
+    float ms = 0.f;
+    auto launch = [&]() {
+        if (kind == 0)
+            hipLaunchKernelGGL(lds_atomic_probe_kernel<double>, dim3(nblk), dim3(256), 0, s, (double*)buf, iters);
+        else if (kind == 1)
+            hipLaunchKernelGGL(lds_atomic_probe_kernel<unsigned>, dim3(nblk), dim3(256), 0, s, (unsigned*)buf, iters);
+        else
+            hipLaunchKernelGGL(global_atomic_probe_kernel, dim3(nblk), dim3(256), 0, s, (float*)buf, gn, greps);
+    };
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = fail(MI355NUFFT_EHIP, "event creation");
+    } else {
+        if (kind == 2 && hipMemsetAsync(buf, 0, bytes, s) != hipSuccess) rc = fail(MI355NUFFT_EHIP, "memset");
+        if (!rc) {
+            launch();  // warm-up
+            if (hipGetLastError() != hipSuccess || hipEventRecord(e0, s) != hipSuccess) {
+                rc = fail(MI355NUFFT_EHIP, "probe launch");
+            } else {
+                launch();
+                if (hipGetLastError() != hipSuccess || hipEventRecord(e1, s) != hipSuccess ||
+                    hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                    rc = fail(MI355NUFFT_EHIP, "probe timing");
+            }
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(buf);
+    if (rc) return rc;
+    // lane operations: LDS 64 lanes x iters per wave, 4 waves per workgroup
+    const double ops = kind == 2 ? (double)gn * greps : (double)nblk * 256.0 * iters;
+    *ops_per_s = ops / (ms * 1e-3);
     return MI355NUFFT_OK;
 }
 

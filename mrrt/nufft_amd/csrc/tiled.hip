@@ -18,6 +18,10 @@ void launch_tiled(bool adjoint, const TiledArgs<T, CK>& a, unsigned nsub, unsign
         if (a.cmax > 0.f) spread = spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED, true>;
     auto interp = a.rc > 1 ? interp_tiled_kernel<T, NDIM, CK, JMAX, FIXED, true>
                            : interp_tiled_kernel<T, NDIM, CK, JMAX, FIXED, false>;
+    if constexpr (FIXED)  // fused tap sums for the J = 6 kernels unless the reference's roundings are asked for
+        if (!a.exact)
+            interp = a.rc > 1 ? interp_tiled_kernel<T, NDIM, CK, JMAX, FIXED, true, true>
+                              : interp_tiled_kernel<T, NDIM, CK, JMAX, FIXED, false, true>;
     if (ev0) {
         if (adjoint)
             hipExtLaunchKernelGGL(spread, dim3(nsub), dim3(threads), (uint32_t)lds, s, ev0, ev1, 0u, a);
@@ -53,6 +57,7 @@ int run_tiled(mi355nufft_plan* p, bool adjoint, const void* samples_in, void* sa
     a.tms = (const T*)p->d_tms;
     a.phs = p->has_phase ? (const cx<T>*)p->d_phs : nullptr;
     a.cmax = (adjoint && use_fxp(p)) ? (float)p->tab_cmax : 0.f;  // > 0: fixed-point scatter tile
+    a.exact = p->exact_taps;
     a.own_ok = 1;
     for (int d = 0; d < p->ndim; ++d)
         if (p->K[d] % p->bg.B[d] != 0) a.own_ok = 0;

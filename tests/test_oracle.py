@@ -273,6 +273,38 @@ def test_phase_before_single_quirk():
     assert 5e-6 < np.sqrt(np.mean(err ** 2)) < 1e-4   # the quirk is present, and bounded
 
 
+@pytest.mark.parametrize("N,J,K,L", [(256, 6, 512, 1024), (128, 6, 256, 512), (65, 6, 130, 256)])
+def test_table_float32_position_quirk(N, J, K, L):
+    """The table is KB at float32-rounded positions (the nested float32
+    NufftBase of _nufft.py:1198-1243: omega cast at :315, t = omega / gamma in
+    float32 at _utils.py:41, :75), not the float32 rounding of KB at the
+    exact positions (which a restatement without the quirk computes exactly:
+    t0 = -J/2 + i/L is exact in float64).  The quirk moves about a quarter of
+    the entries, each by one or two float32 ulps of 1 (the slope of KB times
+    the float32 rounding of the position)."""
+    hq, t0 = O.table_make1("fast", N, J, K, L, "real")
+    a = O.beatty_alpha(J, K, N)
+    naive = O.kaiser_bessel(t0, J, a).astype(np.float32)
+    d = np.abs(hq.astype(np.float64) - naive.astype(np.float64))
+    ulp = float(np.spacing(np.float32(1.0)))
+    assert 0.15 < np.mean(d > 0) < 0.5           # present
+    assert ulp / 2 <= d.max() <= 4 * ulp         # and bounded
+
+
+def test_tm_after_cast_quirk():
+    """tm = fl32(fl32(omega) / fl32(2 pi / K)) in single precision (_nufft.py:315
+    then :338-342), not fl32(omega / (2 pi / K)) in float64: the two differ in
+    the last bits of many samples, by at most 2 ulps of tm."""
+    K = 512
+    om = np.random.default_rng(4).uniform(-np.pi, np.pi, (100000, 1))
+    tq = O.compute_tm(om.astype(np.float32), (K,))[:, 0]
+    naive = (om[:, 0] / (2 * np.pi / K)).astype(np.float32)
+    assert tq.dtype == np.float32
+    diff = np.abs(tq.astype(np.float64) - naive.astype(np.float64))
+    assert np.mean(diff > 0) > 0.2
+    assert np.all(diff <= 2 * np.spacing(np.abs(naive)))
+
+
 # ------------------------------------------------------------- sparse mode
 @pytest.mark.parametrize("phasing", ["real", "complex"])
 @pytest.mark.parametrize("ndim", [1, 2, 3])
