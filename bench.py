@@ -28,14 +28,19 @@ sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
 
-KERNEL_NAMES = {"interp": "interp_tiled_kernel<float, 2, false, 6, true>",
-                "spread": "spread_tiled_kernel<float, 2, false, 6, true>"}
+KERNEL_NAMES = {"interp": "interp_tiled_kernel<float, 2, false, 6, true, false, true>",
+                "spread": "spread_tiled_kernel<float, 2, false, 6, true, false>"}
 
 # HBM bytes per launch of the 2-D 256^2 complex64 kernels, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE passes over this bench (--calibrate-copy), each
 # scaled by what the counter reads for a 256 MiB device copy (FETCH_SIZE
 # x 2.0, WRITE_SIZE x 1.0); profiles/nufft2d_256_c64_hbm_traffic.md.
 TRAFFIC_BYTES = {"spread": 20200960, "interp": 13982822}
+# The same for the 3-D 128^3 / 256^3 kooshball kernels (16^3 bins, fixed-point
+# scatter tile; profiles/nufft3d_128_c64_hbm_traffic.md) and the 24-frame x
+# 32-coil batch (profiles/nufft2d_256_c64_frames_kernel_stats_and_traffic.md).
+TRAFFIC_BYTES_C3 = {"interp": 386.7e6, "spread": 546.2e6}
+TRAFFIC_BYTES_C4 = {"interp": 2613.1e6, "spread": 2364.6e6}
 
 
 def parse():
@@ -266,7 +271,10 @@ def run_c4(steps=3, autotune=True, binning=None, loop=True):
             "c4_interp_bytes_per_transform": int(b_bytes), "c4_autotuned": tuned,
             "c4_gather_kernel_ms": round(g_ms, 4), "c4_scatter_kernel_ms": round(s_ms, 4),
             "c4_gather_GBps": round(b_bytes / (g_ms * 1e-3) / 1e9, 1),
-            "c4_scatter_GBps": round(b_bytes / (s_ms * 1e-3) / 1e9, 1)}
+            "c4_scatter_GBps": round(b_bytes / (s_ms * 1e-3) / 1e9, 1),
+            "c4_traffic_over_algorithmic": {k: round(v / b_bytes, 2) for k, v in TRAFFIC_BYTES_C4.items()},
+            "c4_hbm_GBps_measured_traffic": {"interp": round(TRAFFIC_BYTES_C4["interp"] / (g_ms * 1e-3) / 1e9, 1),
+                                             "spread": round(TRAFFIC_BYTES_C4["spread"] / (s_ms * 1e-3) / 1e9, 1)}}
 
 
 def run_c5(steps=5):
@@ -666,6 +674,9 @@ def main():
         extra["c3_scatter_kernel_ms"] = round(s3, 4)
         extra["c3_gather_GBps"] = round(b3 / (g3 * 1e-3) / 1e9, 1)
         extra["c3_scatter_GBps"] = round(b3 / (s3 * 1e-3) / 1e9, 1)
+        extra["c3_traffic_over_algorithmic"] = {k: round(v / b3, 2) for k, v in TRAFFIC_BYTES_C3.items()}
+        extra["c3_hbm_GBps_measured_traffic"] = {"interp": round(TRAFFIC_BYTES_C3["interp"] / (g3 * 1e-3) / 1e9, 1),
+                                                 "spread": round(TRAFFIC_BYTES_C3["spread"] / (s3 * 1e-3) / 1e9, 1)}
         info3 = r3.A.plan_info()
         acc3 = r3.M * 6 ** 3
         lds_peak = peaks["lds_u32"] if info3["fixed_point_scatter"] else peaks["lds_f64"]
