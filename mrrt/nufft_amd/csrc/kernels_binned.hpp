@@ -706,7 +706,11 @@ __device__ __forceinline__ cx<T> sample_times_conj_phase(const InterpArgs<T, CK>
 // are exact, and the integer sums do not depend on the order of the atomics.
 // Non-finite samples make the whole tile NaN.  Without FXP (and for
 // complex128) the tile accumulates in float64.
-template <typename T, int NDIM, bool CK, int JMAX, bool FIXED, bool FXP>
+//
+// REPS (several repetitions): when a subproblem has at most one point per
+// thread, each thread computes its point's coefficients once and keeps them
+// in registers for every repetition, instead of once per repetition.
+template <typename T, int NDIM, bool CK, int JMAX, bool FIXED, bool FXP, bool REPS = false>
 __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
     static_assert(!FXP || sizeof(T) == 4, "fixed-point accumulation is for complex64");
     extern __shared__ __align__(16) unsigned char smem[];
@@ -723,6 +727,12 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
     for (int sid = blockIdx.x; sid < n_sub; sid += gridDim.x) {
         Sub s;
         load_sub(bg, a.bs, NDIM, sid, s);
+        // REPS: this thread's point (sorted position ic), coefficients cached
+        const bool cached = REPS && s.end - s.start <= (int)blockDim.x;
+        const int ic = s.start + tid;
+        PointTaps<T, CK, JMAX> pc;
+        if constexpr (REPS)
+            if (cached && ic < s.end) point_taps<T, NDIM, CK, JMAX, FIXED>(a, ax, s, ic, pc);
         for (int r = 0; r < ia.nreps; ++r) {
             for (int e = tid; e < bg.tile; e += blockDim.x) {
                 tre[e] = 0;
@@ -744,17 +754,24 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
                 sh = 31 - e;
             }
             __syncthreads();
-            for (int i = s.start + tid; i < s.end; i += blockDim.x) {
-                PointTaps<T, CK, JMAX> pt;
-                point_taps<T, NDIM, CK, JMAX, FIXED>(a, ax, s, i, pt);
+            auto add_point = [&](const PointTaps<T, CK, JMAX>& pt) {
                 cx<T> f = sample_times_conj_phase<T, CK, JMAX>(ia, r, pt, a.phs);
                 if constexpr (FXP) {
-                    if (bad) continue;
+                    if (bad) return;
                     f.x = ldexpf(f.x, sh);
                     f.y = ldexpf(f.y, sh);
                     scatter_taps_fx<NDIM, CK, JMAX, FIXED>(tre, tim, pt, f, ax, T0, T1);
                 } else {
                     scatter_taps<T, NDIM, CK, JMAX, FIXED>(tre, tim, pt, f, ax, T0, T1);
+                }
+            };
+            if (cached) {
+                if (ic < s.end) add_point(pc);
+            } else {
+                for (int i = s.start + tid; i < s.end; i += blockDim.x) {
+                    PointTaps<T, CK, JMAX> pt;
+                    point_taps<T, NDIM, CK, JMAX, FIXED>(a, ax, s, i, pt);
+                    add_point(pt);
                 }
             }
             __syncthreads();

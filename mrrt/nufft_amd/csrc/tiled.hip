@@ -13,9 +13,13 @@ namespace mi355host {
 template <typename T, bool CK, int NDIM, int JMAX, bool FIXED>
 void launch_tiled(bool adjoint, const TiledArgs<T, CK>& a, unsigned nsub, unsigned threads, size_t lds,
                   hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
-    auto spread = spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED, false>;
+    const bool reps = a.ia.nreps > 1;  // coefficients cached across repetitions
+    auto spread = reps ? spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED, false, true>
+                       : spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED, false, false>;
     if constexpr (sizeof(T) == 4)
-        if (a.cmax > 0.f) spread = spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED, true>;
+        if (a.cmax > 0.f)
+            spread = reps ? spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED, true, true>
+                          : spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED, true, false>;
     auto interp = a.rc > 1 ? interp_tiled_kernel<T, NDIM, CK, JMAX, FIXED, true>
                            : interp_tiled_kernel<T, NDIM, CK, JMAX, FIXED, false>;
     if constexpr (FIXED)  // fused tap sums for the J = 6 kernels unless the reference's roundings are asked for
