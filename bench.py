@@ -191,16 +191,31 @@ def measure_atomic_peaks():
     return out
 
 
-def kernel_means(op, fn, n):
+def kernel_means(op, fn, n, atomics=None):
     """(gather_ms, scatter_ms): mean durations of the tiled kernels over n
-    calls of fn, from the plan's dispatch-timestamped HIP events."""
+    calls of fn, from the plan's dispatch-timestamped HIP events.  A dict
+    passed as ``atomics`` receives the scatter's flush atomics per launch
+    (global float atomic lane operations, counted in the kernel)."""
     op.set_kernel_timing(True)
+    op.read_flush_atomics()
     for _ in range(n):
         fn()
-    _, g = op.read_kernel_timing("interp")
-    _, sc = op.read_kernel_timing("spread")
+    ng, g = op.read_kernel_timing("interp")
+    ns, sc = op.read_kernel_timing("spread")
+    if atomics is not None:
+        atomics["per_launch"] = op.read_flush_atomics() / max(ns, 1)
     op.set_kernel_timing(False)
     return g, sc
+
+
+def flush_atomic_rate(atomics_per_launch, scatter_ms, peak):
+    """The flush's global atomic lane operations per second over the whole
+    scatter launch (a lower bound of the flush's own rate) and its fraction
+    of the measured global float atomic rate."""
+    rate = atomics_per_launch / (scatter_ms * 1e-3)
+    return {"lane_ops_per_launch": int(atomics_per_launch), "lane_ops_per_s": round(rate, 1),
+            "peak_lane_ops_per_s": round(peak, 1), "frac": round(rate / peak, 4),
+            "peak_basis": "global_atomic_add_f32, coalesced, measured here (mi355nufft_measure_atomics kind 2)"}
 
 
 def run_c4(steps=3, autotune=True, binning=None, loop=True):
@@ -260,7 +275,8 @@ def run_c4(steps=3, autotune=True, binning=None, loop=True):
     xc = torch.randn((F, C) + Nd, dtype=torch.complex64, device=dev)
     yc = torch.empty((C, B.M), dtype=torch.complex64, device=dev)
     xac = torch.empty_like(xc)
-    g_ms, s_ms = kernel_means(B.op, lambda: (B.forward(xc, out=yc), B.adjoint(yc, out=xac)), steps)
+    at4 = {}
+    g_ms, s_ms = kernel_means(B.op, lambda: (B.forward(xc, out=yc), B.adjoint(yc, out=xac)), steps, at4)
     b_bytes = sum(interp_bytes(om.shape[0], 2, 512 * 512, C, 4) for om in frames)
     del graph
     return {"c4_frames_x_coils": [F, C], "c4_M_total": int(B.M),
@@ -270,6 +286,7 @@ def run_c4(steps=3, autotune=True, binning=None, loop=True):
             "c4_transforms_per_s_graph": round(2 * F * C * 1000.0 / t_b, 1),
             "c4_interp_bytes_per_transform": int(b_bytes), "c4_autotuned": tuned,
             "c4_gather_kernel_ms": round(g_ms, 4), "c4_scatter_kernel_ms": round(s_ms, 4),
+            "c4_scatter_flush_atomics_per_launch": int(at4["per_launch"]),
             "c4_gather_GBps": round(b_bytes / (g_ms * 1e-3) / 1e9, 1),
             "c4_scatter_GBps": round(b_bytes / (s_ms * 1e-3) / 1e9, 1),
             "c4_traffic_over_algorithmic": {k: round(v / b_bytes, 2) for k, v in TRAFFIC_BYTES_C4.items()},
@@ -595,9 +612,11 @@ def main():
     # gather / scatter launch carries a HIP event pair timestamped by its
     # dispatch on the plan's stream (the kernel's execution only).
     run.A.set_kernel_timing(True)
+    run.A.read_flush_atomics()
     _time_fn(run.pair, args.steps, 0)
-    run.A.set_kernel_timing(False)
     kern = {w: run.A.read_kernel_timing(w) for w in ("interp", "spread")}
+    flush_atomics = run.A.read_flush_atomics() / max(kern["spread"][0], 1)
+    run.A.set_kernel_timing(False)
     print("bench: kernel means (ms) %s" % json.dumps({k: [v[0], round(v[1], 5)] for k, v in kern.items()}),
           file=sys.stderr, flush=True)
     b = interp_bytes(run.M, 2, run.nk, 1, 4)
@@ -628,6 +647,9 @@ def main():
         "peak_basis": "ds_add_f64 at scattered addresses, measured here (mi355nufft_measure_atomics kind 0)",
     }
     roofline["atomic_peaks_lane_ops_per_s"] = {k: round(v, 1) for k, v in peaks.items()}
+    # the flush's global float atomics (counted in the kernel) against the
+    # measured global float atomic rate
+    roofline["scatter_flush_global_atomics"] = flush_atomic_rate(flush_atomics, kern["spread"][1], peaks["global_f32"])
     extra = {}
     if not args.no_extra:
         other = "rocfft" if fft_choice == "auto" else "auto"
@@ -666,7 +688,8 @@ def main():
         r3 = GpuRun(c3_workload())
         t3 = r3.time(r3.pair, 5, 1)
         st3 = r3.stage_times(5, 1)
-        g3, s3 = kernel_means(r3.A, r3.pair, 5)
+        at3 = {}
+        g3, s3 = kernel_means(r3.A, r3.pair, 5, at3)
         extra["c3_3d_complex64_pairs_per_s"] = round(1000.0 / t3, 3)
         extra["c3_stages_ms"] = {k: round(v, 4) for k, v in st3.items()}
         b3 = interp_bytes(r3.M, 3, r3.nk, 1, 4)
@@ -681,13 +704,17 @@ def main():
         acc3 = r3.M * 6 ** 3
         lds_peak = peaks["lds_u32"] if info3["fixed_point_scatter"] else peaks["lds_f64"]
         extra["c3_scatter_lds_atomic_frac"] = round(2 * acc3 / (s3 * 1e-3) / lds_peak, 4)
+        extra["c3_scatter_flush_global_atomics"] = flush_atomic_rate(at3["per_launch"], s3, peaks["global_f32"])
         extra["c3_scatter_accumulation"] = "fixed32" if info3["fixed_point_scatter"] else "float64"
         del r3
         torch.cuda.empty_cache()
         extra.update(run_sparse(c3_workload(), "c3", 5, 1, "auto"))
         torch.cuda.empty_cache()
         print("bench: frames batch (24 x 32)", file=sys.stderr, flush=True)
-        extra.update(run_c4())
+        c4 = run_c4()
+        c4["c4_scatter_flush_global_atomics"] = flush_atomic_rate(
+            c4["c4_scatter_flush_atomics_per_launch"], c4["c4_scatter_kernel_ms"], peaks["global_f32"])
+        extra.update(c4)
         torch.cuda.empty_cache()
         print("bench: autograd spiral", file=sys.stderr, flush=True)
         extra.update(run_c5(steps=20))

@@ -164,6 +164,16 @@ int mi355nufft_read_timing(mi355nufft_plan *plan, int32_t which, int64_t *launch
                            double *total_ms);
 
 /*
+ * Global atomic lane operations (float atomic adds, one per component of a
+ * flushed tile cell) made by the tiled scatter launches timed since the last
+ * call, then reset.  Counted in the kernel while timing is enabled; waits for
+ * the plan's stream.  The flush's atomic rate is this count over the
+ * scatter's timed duration.  Instrumentation for benchmarks; no counterpart
+ * in the reference, whose scatter makes two atomics per tap.
+ */
+int mi355nufft_read_flush_atomics(mi355nufft_plan *plan, uint64_t *lane_ops);
+
+/*
  * Accumulation of the tiled scatter's LDS tile (no counterpart in the
  * reference, whose scatter accumulates in the operator's precision):
  *   accumulate  0 auto: 3-D complex64 plans accumulate in 32-bit fixed point

@@ -48,6 +48,7 @@ _SIGNATURES = {
     "mi355nufft_set_tiling": (c_int, [c_vp, c_i32p, c_i32, c_int, c_i32]),
     "mi355nufft_set_timing": (c_int, [c_vp, c_i32]),
     "mi355nufft_read_timing": (c_int, [c_vp, c_i32, c_i64p, c_dblp]),
+    "mi355nufft_read_flush_atomics": (c_int, [c_vp, ctypes.POINTER(ctypes.c_uint64)]),
     "mi355nufft_plan_info": (c_int, [c_vp, c_i64p, c_i32]),
     "mi355nufft_forward": (c_int, [c_vp, c_vp, c_i64p, c_vp, c_i32, c_vp]),
     "mi355nufft_adjoint": (c_int, [c_vp, c_vp, c_vp, c_i64p, c_i32, c_vp]),

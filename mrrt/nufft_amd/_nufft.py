@@ -448,6 +448,13 @@ class NufftBase(object):
         _lib.check(self._lib.mi355nufft_read_timing(self._plan, w, ctypes.byref(n), ctypes.byref(ms)))
         return n.value, (ms.value / n.value if n.value else float("nan"))
 
+    def read_flush_atomics(self):
+        """Global float atomic lane operations made by the tile flushes of the
+        scatter launches timed since the last read (then reset)."""
+        v = ctypes.c_uint64(0)
+        _lib.check(self._lib.mi355nufft_read_flush_atomics(self._plan, ctypes.byref(v)))
+        return v.value
+
     def _h_dev(self):
         if getattr(self, "_h_dev_cache", None) is None:
             self._h_dev_cache = [torch.from_numpy(t).to(self.device) for t in self._tables_host]

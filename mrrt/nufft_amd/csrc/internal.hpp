@@ -174,6 +174,10 @@ struct mi355nufft_plan {
     int tev_used[2] = {0, 0};
     double tev_ms[2] = {0, 0};       // folded totals of earlier pairs
     int64_t tev_n[2] = {0, 0};
+    // global atomic lane operations of the scatter's tile flushes while
+    // timing is on (mi355nufft_read_flush_atomics), kFlushSlots partial
+    // counts summed on the host
+    unsigned long long* d_flush_count = nullptr;
 
     SparseState sp;
 
@@ -187,6 +191,10 @@ struct mi355nufft_plan {
 };
 
 namespace mi355host {
+
+// Partial counters of the flush atomic count: workgroup b adds to slot
+// b mod kFlushSlots, so the counting adds no same-address contention.
+constexpr int kFlushSlots = 1024;
 
 // Whether the scatter tile accumulates complex64 in 32-bit fixed point (see
 // spread_tiled_kernel): by default in 3-D, where it measured 26% faster at

@@ -244,6 +244,8 @@ struct TiledArgs {
     int own_ok;             // scatter: every K_d is a multiple of B_d (see spread_tiled_kernel)
     float cmax;             // scatter: > 0 accumulates in fixed point; bound of |coefficient product|
     int exact;              // gather: 1 keeps the reference's roundings (no fused multiply-add)
+    unsigned long long* flush_count;  // scatter: non-null adds the flush's global atomic lane operations
+    int flush_slots;                  // power of two: workgroup b adds to flush_count[b % flush_slots]
 };
 
 // Copy tm and the per-sample phase into sorted (bin) order, so the tiled
@@ -724,6 +726,7 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
     const int T0 = bg.T[0], T1 = bg.T[1];
     const int tid = threadIdx.x;
     const int n_sub = a.bs.n_sub[0];
+    unsigned natom = 0;  // global atomic lane operations of this thread's flushes
     for (int sid = blockIdx.x; sid < n_sub; sid += gridDim.x) {
         Sub s;
         load_sub(bg, a.bs, NDIM, sid, s);
@@ -832,10 +835,16 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
                     T* q = reinterpret_cast<T*>(g + gi);
                     unsafeAtomicAdd(q, vr);
                     unsafeAtomicAdd(q + 1, vi);
+                    natom += 2;
                 }
             }
             __syncthreads();
         }
+    }
+    if (a.flush_count) {  // uniform: one 64-bit atomic per wave, into this workgroup's slot
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) natom += __shfl_xor(natom, o);
+        if ((tid & 63) == 0) atomicAdd(a.flush_count + (blockIdx.x & (a.flush_slots - 1)), (unsigned long long)natom);
     }
 }
 

@@ -1394,6 +1394,7 @@ int mi355nufft_plan_destroy(mi355nufft_plan* p) {
     if (p->d_sub) (void)hipFree(p->d_sub);
     if (p->d_frame_off) (void)hipFree(p->d_frame_off);
     if (p->d_tiles) (void)hipFree(p->d_tiles);
+    if (p->d_flush_count) (void)hipFree(p->d_flush_count);
     if (p->d_tms) (void)hipFree(p->d_tms);
     if (p->d_phs) (void)hipFree(p->d_phs);
     for (auto& v : p->tev)
@@ -1526,7 +1527,30 @@ int mi355nufft_set_tiling(mi355nufft_plan* p, const int32_t* bin_size, int32_t m
 
 int mi355nufft_set_timing(mi355nufft_plan* p, int32_t enable) {
     if (!p) return fail(MI355NUFFT_EINVAL, "null plan");
+    if (enable && !p->d_flush_count) {
+        const size_t bytes = kFlushSlots * sizeof(unsigned long long);
+        if (hipMalloc(&p->d_flush_count, bytes) != hipSuccess) {
+            p->d_flush_count = nullptr;
+            return fail(MI355NUFFT_ENOMEM, "flush counters");
+        }
+        HIP_TRY(hipMemsetAsync(p->d_flush_count, 0, bytes, p->stream));
+    }
     p->timing = enable ? 1 : 0;
+    return MI355NUFFT_OK;
+}
+
+int mi355nufft_read_flush_atomics(mi355nufft_plan* p, uint64_t* lane_ops) {
+    if (!p || !lane_ops) return fail(MI355NUFFT_EINVAL, "null argument");
+    *lane_ops = 0;
+    if (!p->d_flush_count) return MI355NUFFT_OK;
+    std::vector<unsigned long long> v(kFlushSlots, 0);
+    const size_t bytes = kFlushSlots * sizeof(unsigned long long);
+    HIP_TRY(hipMemcpyAsync(v.data(), p->d_flush_count, bytes, hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    HIP_TRY(hipMemsetAsync(p->d_flush_count, 0, bytes, p->stream));
+    uint64_t total = 0;
+    for (unsigned long long c : v) total += c;
+    *lane_ops = total;
     return MI355NUFFT_OK;
 }
 

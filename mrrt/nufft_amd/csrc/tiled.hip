@@ -80,6 +80,8 @@ int run_tiled(mi355nufft_plan* p, bool adjoint, const void* samples_in, void* sa
     const int jf = fixed_j(p->ndim, J);
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     RET_IF(timing_events(p, adjoint ? 1 : 0, &ev0, &ev1));
+    a.flush_count = (adjoint && ev0) ? p->d_flush_count : nullptr;  // counted with the timed launches
+    a.flush_slots = kFlushSlots;
 #define MI355_FCASE(ND, JF)                                                    \
     if (p->ndim == ND && jf == JF) {                                           \
         launch_tiled<T, CK, ND, JF, true>(adjoint, a, nsub, p->bg_threads, lds, p->stream, ev0, ev1); \

@@ -515,6 +515,30 @@ def test_kernel_timing_events():
         A.read_kernel_timing("fft")
 
 
+def test_flush_atomic_count():
+    """The scatter's flush makes one float atomic per component of each
+    non-zero tile cell outside its bin's own cells, which get plain stores
+    (local index J-1 <= e < B on every axis, 16x16 bins, J = 6).  A point whose
+    taps start at local cell (10, 3) covers 6 x 6 cells, 6 x 4 of them its
+    bin's own: 12 cells, 24 atomics per repetition.  One starting at (0, 0)
+    of another bin has a single own cell: 35 cells, 70 atomics.  Counted only
+    while kernel timing is on."""
+    Nd, Kd = (64, 64), (128, 128)
+    t = np.array([[28.5, 21.5], [34.5, 34.5]])  # first taps (26, 19) and (32, 32)
+    A = NufftBase(Nd, t * 2 * np.pi / 128, Jd=6, Kd=Kd, Ld=1024, device=dev())
+    assert A.plan_info()["bin_size"][:2] == (16, 16)
+    y = torch.ones((3, 2), dtype=A._tcdt, device=dev())
+    A._adjoint_dev(y, 3)
+    assert A.read_flush_atomics() == 0
+    A.set_kernel_timing(True)
+    A._adjoint_dev(y, 3)
+    A._adjoint_dev(y[:1].contiguous(), 1)
+    assert A.read_kernel_timing("spread")[0] == 2
+    assert A.read_flush_atomics() == 4 * (24 + 70)
+    assert A.read_flush_atomics() == 0
+    A.set_kernel_timing(False)
+
+
 @pytest.mark.parametrize("precision", ["single", "double"])
 @pytest.mark.parametrize("phasing", ["real", "complex"])
 def test_grid_only_and_psf_vs_oracle(precision, phasing):
