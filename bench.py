@@ -197,15 +197,27 @@ def kernel_means(op, fn, n, atomics=None):
     passed as ``atomics`` receives the scatter's flush atomics per launch
     (global float atomic lane operations, counted in the kernel)."""
     op.set_kernel_timing(True)
+    for _ in range(n):
+        fn()
+    _, g = op.read_kernel_timing("interp")
+    _, sc = op.read_kernel_timing("spread")
+    if atomics is not None:  # a separate run: the count slightly lengthens short launches
+        atomics["per_launch"] = flush_atomics_per_launch(op, fn, n)
+    op.set_kernel_timing(False)
+    return g, sc
+
+
+def flush_atomics_per_launch(op, fn, n):
+    """Global float atomic lane operations of one scatter launch's tile
+    flushes, counted in the kernel over n calls of fn."""
+    op.set_kernel_timing(True, count_atomics=True)
     op.read_flush_atomics()
     for _ in range(n):
         fn()
-    ng, g = op.read_kernel_timing("interp")
-    ns, sc = op.read_kernel_timing("spread")
-    if atomics is not None:
-        atomics["per_launch"] = op.read_flush_atomics() / max(ns, 1)
+    ns, _ = op.read_kernel_timing("spread")
+    v = op.read_flush_atomics() / max(ns, 1)
     op.set_kernel_timing(False)
-    return g, sc
+    return v
 
 
 def flush_atomic_rate(atomics_per_launch, scatter_ms, peak):
@@ -612,11 +624,10 @@ def main():
     # gather / scatter launch carries a HIP event pair timestamped by its
     # dispatch on the plan's stream (the kernel's execution only).
     run.A.set_kernel_timing(True)
-    run.A.read_flush_atomics()
     _time_fn(run.pair, args.steps, 0)
-    kern = {w: run.A.read_kernel_timing(w) for w in ("interp", "spread")}
-    flush_atomics = run.A.read_flush_atomics() / max(kern["spread"][0], 1)
     run.A.set_kernel_timing(False)
+    kern = {w: run.A.read_kernel_timing(w) for w in ("interp", "spread")}
+    flush_atomics = flush_atomics_per_launch(run.A, run.pair, 10)
     print("bench: kernel means (ms) %s" % json.dumps({k: [v[0], round(v[1], 5)] for k, v in kern.items()}),
           file=sys.stderr, flush=True)
     b = interp_bytes(run.M, 2, run.nk, 1, 4)

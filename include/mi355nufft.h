@@ -166,8 +166,8 @@ int mi355nufft_read_timing(mi355nufft_plan *plan, int32_t which, int64_t *launch
 /*
  * Global atomic lane operations (float atomic adds, one per component of a
  * flushed tile cell) made by the tiled scatter launches timed since the last
- * call, then reset.  Counted in the kernel while timing is enabled; waits for
- * the plan's stream.  The flush's atomic rate is this count over the
+ * call, then reset.  Counted in the kernel while timing is enabled with bit 1
+ * set (mi355nufft_set_timing(plan, 3)); waits for the plan's stream.  The flush's atomic rate is this count over the
  * scatter's timed duration.  Instrumentation for benchmarks; no counterpart
  * in the reference, whose scatter makes two atomics per tap.
  */

@@ -434,10 +434,13 @@ class NufftBase(object):
         m = {"auto": 0, "rocfft": 1}[method]
         _lib.check(self._lib.mi355nufft_set_fft(self._plan, m))
 
-    def set_kernel_timing(self, enable=True):
+    def set_kernel_timing(self, enable=True, count_atomics=False):
         """Bracket every tiled gather / scatter launch with HIP events on the
-        plan's stream (skipped while the stream is being captured)."""
-        _lib.check(self._lib.mi355nufft_set_timing(self._plan, 1 if enable else 0))
+        plan's stream (skipped while the stream is being captured);
+        ``count_atomics`` also counts the scatter flush's global atomics
+        (``read_flush_atomics``)."""
+        v = (3 if count_atomics else 1) if enable else 0
+        _lib.check(self._lib.mi355nufft_set_timing(self._plan, v))
 
     def read_kernel_timing(self, which):
         """(launches, mean_ms) of the gather ("interp") or scatter ("spread")

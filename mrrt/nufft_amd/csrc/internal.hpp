@@ -168,7 +168,8 @@ struct mi355nufft_plan {
     int64_t* d_frame_off = nullptr;
     int64_t frame_cap = 0;
 
-    // launch timing of the tiled gather (0) and scatter (1) kernels
+    // launch timing of the tiled gather (0) and scatter (1) kernels; bit 1:
+    // also count the scatter's flush atomics
     int timing = 0;
     std::vector<hipEvent_t> tev[2];  // start/stop pairs, reused
     int tev_used[2] = {0, 0};

@@ -1527,7 +1527,7 @@ int mi355nufft_set_tiling(mi355nufft_plan* p, const int32_t* bin_size, int32_t m
 
 int mi355nufft_set_timing(mi355nufft_plan* p, int32_t enable) {
     if (!p) return fail(MI355NUFFT_EINVAL, "null plan");
-    if (enable && !p->d_flush_count) {
+    if ((enable & 2) && !p->d_flush_count) {
         const size_t bytes = kFlushSlots * sizeof(unsigned long long);
         if (hipMalloc(&p->d_flush_count, bytes) != hipSuccess) {
             p->d_flush_count = nullptr;
@@ -1535,7 +1535,7 @@ int mi355nufft_set_timing(mi355nufft_plan* p, int32_t enable) {
         }
         HIP_TRY(hipMemsetAsync(p->d_flush_count, 0, bytes, p->stream));
     }
-    p->timing = enable ? 1 : 0;
+    p->timing = enable ? (enable & 3) | 1 : 0;
     return MI355NUFFT_OK;
 }
 

@@ -80,7 +80,9 @@ int run_tiled(mi355nufft_plan* p, bool adjoint, const void* samples_in, void* sa
     const int jf = fixed_j(p->ndim, J);
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     RET_IF(timing_events(p, adjoint ? 1 : 0, &ev0, &ev1));
-    a.flush_count = (adjoint && ev0) ? p->d_flush_count : nullptr;  // counted with the timed launches
+    // counted with the timed launches when asked for (the count's final
+    // atomics lengthen a short launch, so timing alone leaves it out)
+    a.flush_count = (adjoint && ev0 && (p->timing & 2)) ? p->d_flush_count : nullptr;
     a.flush_slots = kFlushSlots;
 #define MI355_FCASE(ND, JF)                                                    \
     if (p->ndim == ND && jf == JF) {                                           \

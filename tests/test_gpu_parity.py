@@ -522,7 +522,7 @@ def test_flush_atomic_count():
     taps start at local cell (10, 3) covers 6 x 6 cells, 6 x 4 of them its
     bin's own: 12 cells, 24 atomics per repetition.  One starting at (0, 0)
     of another bin has a single own cell: 35 cells, 70 atomics.  Counted only
-    while kernel timing is on."""
+    while kernel timing is on with the count asked for."""
     Nd, Kd = (64, 64), (128, 128)
     t = np.array([[28.5, 21.5], [34.5, 34.5]])  # first taps (26, 19) and (32, 32)
     A = NufftBase(Nd, t * 2 * np.pi / 128, Jd=6, Kd=Kd, Ld=1024, device=dev())
@@ -531,6 +531,10 @@ def test_flush_atomic_count():
     A._adjoint_dev(y, 3)
     assert A.read_flush_atomics() == 0
     A.set_kernel_timing(True)
+    A._adjoint_dev(y, 3)
+    assert A.read_flush_atomics() == 0
+    A.read_kernel_timing("spread")
+    A.set_kernel_timing(True, count_atomics=True)
     A._adjoint_dev(y, 3)
     A._adjoint_dev(y[:1].contiguous(), 1)
     assert A.read_kernel_timing("spread")[0] == 2
