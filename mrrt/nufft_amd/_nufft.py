@@ -14,6 +14,7 @@ reference's CPU path) or torch tensors (results stay on the device).
 """
 
 import ctypes
+import threading
 import warnings
 from math import sqrt
 
@@ -602,21 +603,49 @@ def _to_device(obj, x):
     return t, was_numpy
 
 
-def _to_output(t, was_numpy):
-    """Device results for numpy callers: a new numpy array (synchronous).
+class _OutStaging(object):
+    """Persistent page-locked landing buffer of one device for results going
+    back to numpy callers, grown on demand.  A result arrives in it by one
+    DMA at pinned rate and is then copied into an ordinary numpy array, so
+    callers never hold page-locked memory."""
 
-    Dense results (any dimension order) land in page-locked memory from
-    torch's caching host allocator with the device tensor's strides, so the
-    copy is one DMA at pinned rate; the array keeps that block until it is
-    freed, and the allocator then reuses it.  Other layouts take an ordinary
-    copy."""
+    def __init__(self):
+        self.lock = threading.Lock()
+        self.buf = None
+
+    def download(self, t):
+        nbytes = t.numel() * t.element_size()
+        with self.lock:
+            if self.buf is None or self.buf.numel() < nbytes:
+                self.buf = torch.empty(max(nbytes, 1 << 20), dtype=torch.uint8, pin_memory=True)
+            flat = self.buf[:nbytes].view(t.dtype)
+            if t.is_contiguous():
+                staged = flat.reshape(t.shape)
+            else:  # Fortran order
+                staged = flat.reshape(tuple(reversed(t.shape))).permute(*reversed(range(t.dim())))
+            staged.copy_(t, non_blocking=True)
+            torch.cuda.current_stream(t.device).synchronize()
+            return np.array(staged.numpy(), copy=True, order="K")
+
+
+_OUT_STAGING = {}
+_OUT_STAGING_LOCK = threading.Lock()
+
+
+def _to_output(t, was_numpy):
+    """Device results for numpy callers: a new ordinary numpy array
+    (synchronous).  Dense results (C or Fortran order) go through the
+    device's persistent pinned landing buffer and keep their memory order;
+    other layouts take a plain device-to-host copy."""
     if not was_numpy:
         return t
     if t.is_contiguous() or t.permute(*reversed(range(t.dim()))).is_contiguous():
-        out = torch.empty_strided(t.shape, t.stride(), dtype=t.dtype, pin_memory=True)
-        out.copy_(t, non_blocking=True)
-        torch.cuda.current_stream(t.device).synchronize()
-        return out.numpy()
+        key = t.device.index if t.device.index is not None else torch.cuda.current_device()
+        with _OUT_STAGING_LOCK:
+            st = _OUT_STAGING.get(key)
+            if st is None:
+                st = _OUT_STAGING[key] = _OutStaging()
+        return st.download(t)
     return t.cpu().numpy()
 
 
