@@ -42,6 +42,10 @@ struct FftCfg {
     // different banks
     static constexpr int KP = K + K / 32 + 1;
     static constexpr int TL = K / 8 < 64 ? K / 8 : 64;
+    // each thread loads NL = K / TL points of a line, LF of them in flight
+    // at once: the loads of a batch are all issued before any is used
+    static constexpr int NL = K / TL;
+    static constexpr int LF = NL < 8 ? NL : 8;
     static constexpr size_t lds_bytes(int lpw) { return (size_t)(lpw * KP + K) * CXS; }
 };
 
@@ -249,15 +253,26 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_rows_pad_kernel(const cx<T
     const bool mlive = mline < g.nlines;
     if (g.across && mlive) row_of_line(mline, g, m2, m3, mr);
     cx<T>* lm = reinterpret_cast<cx<T>*>(smem) + lj * C::KP;
-    for (int t = t0; t < K; t += TL) {
-        cx<T> o = mkcx<T>((T)0, (T)0);
-        if (mlive && t < g.N[0]) {
-            const T s = sn0 ? deapod_factor<T>(sn, ndim, t, m2, m3) : (T)1;
-            const cx<T> v = x[(int64_t)t * g.xs[0] + (int64_t)m2 * g.xs[1] + (int64_t)m3 * g.xs[2] + mr * g.xs[3]];
-            o.x = v.x * s;
-            o.y = v.y * s;
+    const cx<T>* xl = x + (int64_t)m2 * g.xs[1] + (int64_t)m3 * g.xs[2] + mr * g.xs[3];
+#pragma unroll
+    for (int kb = 0; kb < C::NL; kb += C::LF) {
+        cx<T> v[C::LF];
+#pragma unroll
+        for (int u = 0; u < C::LF; ++u) {
+            const int t = t0 + (kb + u) * TL;
+            v[u] = (mlive && t < g.N[0]) ? xl[(int64_t)t * g.xs[0]] : mkcx<T>((T)0, (T)0);
         }
-        lm[pad(t)] = o;
+#pragma unroll
+        for (int u = 0; u < C::LF; ++u) {
+            const int t = t0 + (kb + u) * TL;
+            cx<T> o = mkcx<T>((T)0, (T)0);
+            if (mlive && t < g.N[0]) {
+                const T s = sn0 ? deapod_factor<T>(sn, ndim, t, m2, m3) : (T)1;
+                o.x = v[u].x * s;
+                o.y = v[u].y * s;
+            }
+            lm[pad(t)] = o;
+        }
     }
     cx<T>* tws = stage_twiddles<T, K>(smem, g.lpw, tw);  // after the data loads are in flight
     __syncthreads();
@@ -290,10 +305,20 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_rows_crop_kernel(cx<T>* __
     cx<T>* ln = reinterpret_cast<cx<T>*>(smem) + li * C::KP;
     cx<T>* in = grid + r * g.nk + ((int64_t)n3 * g.K[1] + n2) * K;
     const cx<T>* mul = g.mult ? (const cx<T>*)g.mult + ((int64_t)n3 * g.K[1] + n2) * K : nullptr;
-    for (int t = tl; t < K; t += TL) {
-        cx<T> v = live ? in[t] : mkcx<T>((T)0, (T)0);
-        if (mul && live) v = cmulc<T>(v, mul[t]);
-        ln[pad(t)] = v;
+#pragma unroll
+    for (int kb = 0; kb < C::NL; kb += C::LF) {
+        cx<T> v[C::LF], w[C::LF];
+#pragma unroll
+        for (int u = 0; u < C::LF; ++u) {
+            const int t = tl + (kb + u) * TL;
+            v[u] = live ? in[t] : mkcx<T>((T)0, (T)0);
+            if (mul && live) w[u] = mul[t];
+        }
+#pragma unroll
+        for (int u = 0; u < C::LF; ++u) {
+            const int t = tl + (kb + u) * TL;
+            ln[pad(t)] = (mul && live) ? cmulc<T>(v[u], w[u]) : v[u];
+        }
     }
     cx<T>* tws = stage_twiddles<T, K>(smem, g.lpw, tw);
     __syncthreads();
@@ -397,14 +422,25 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_cols_kernel(cx<T>* __restr
     const int li = threadIdx.x % lpw, t0 = threadIdx.x / lpw;  // load / store mapping
     const cx<T>* mul = (const cx<T>*)g.mult;
     const int64_t i0 = chunk * lpw + li;  // inner index of this thread's line
-    for (int t = t0; t < K; t += TL) {
-        cx<T> v = mkcx<T>((T)0, (T)0);
-        if (t < g.n_in) {
-            v = grid[base + (int64_t)t * g.S + li];
-            if (mul) v = cmulc<T>(v, mul[ibase + (int64_t)t * g.S + li]);
-            if (g.pb == 2) v = fft_pb_apply<T>(v, g, i0, t, true);
+#pragma unroll
+    for (int kb = 0; kb < C::NL; kb += C::LF) {
+        cx<T> v[C::LF], w[C::LF];
+#pragma unroll
+        for (int u = 0; u < C::LF; ++u) {
+            const int t = t0 + (kb + u) * TL;
+            v[u] = t < g.n_in ? grid[base + (int64_t)t * g.S + li] : mkcx<T>((T)0, (T)0);
+            if (mul && t < g.n_in) w[u] = mul[ibase + (int64_t)t * g.S + li];
         }
-        lds[li * C::KP + pad(t)] = v;
+#pragma unroll
+        for (int u = 0; u < C::LF; ++u) {
+            const int t = t0 + (kb + u) * TL;
+            cx<T> x = v[u];
+            if (t < g.n_in) {
+                if (mul) x = cmulc<T>(x, w[u]);
+                if (g.pb == 2) x = fft_pb_apply<T>(x, g, i0, t, true);
+            }
+            lds[li * C::KP + pad(t)] = x;
+        }
     }
     cx<T>* tws = stage_twiddles<T, K>(smem, g.lpw, tw);
     __syncthreads();

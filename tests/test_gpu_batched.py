@@ -50,11 +50,15 @@ def test_frames_vs_oracle_per_frame(precision, phasing):
         assert rel_l2(xan[f], ref_x) <= TOL[precision]
 
 
-def test_frames_graph_replays_match_eager():
+@pytest.mark.parametrize("Nd,Kd", [((48, 48), (96, 96)), ((64, 64), (128, 128))])
+def test_frames_graph_replays_match_eager(Nd, Kd):
     """Replay the captured forward/adjoint several times with eager work and
-    synchronisations in between; every replay equals the eager result."""
-    Nd, Kd = (48, 48), (96, 96)
-    frames = trj.golden_angle_frames(5, 96, smin=4, smax=12, seed=4)
+    synchronisations in between; every replay equals the eager result.  Kd
+    96^2 runs rocFFT with the re-zeroing crop, 128^2 the fused FFT passes,
+    whose inverse passes leave the plan's adjoint grid zero (the captured
+    adjoint skips clearing it): an eager adjoint of zero samples after the
+    replays must return exact zeros."""
+    frames = trj.golden_angle_frames(5, Kd[0], smin=4, smax=12, seed=4)
     C = 2
     B = NufftFrames(Nd, frames, C, Jd=6, Kd=Kd, Ld=1024, device=dev())
     x = torch.from_numpy(trj.complex_gaussian((5, C) + Nd, seed=5)).to(dev()).to(torch.complex64)
@@ -77,6 +81,11 @@ def test_frames_graph_replays_match_eager():
         xe = B.adjoint(y_ref)
         assert rel_l2(xe.cpu().numpy(), xa_ref.cpu().numpy()) <= 1e-6
         _ = (x * 2).sum().item()      # eager work between replays
+    B.replay_forward()
+    B.replay_adjoint()
+    torch.cuda.synchronize()
+    z = B.adjoint(torch.zeros_like(y_ref))
+    assert torch.count_nonzero(z).item() == 0
 
 
 def test_frames_match_separate_plans():
