@@ -402,6 +402,22 @@ __device__ __forceinline__ void point_taps(const TiledArgs<T, CK>& a, const Axis
     }
 }
 
+// point_taps with the point's positions, index and phase already loaded
+template <typename T, int NDIM, bool CK, int JMAX, bool FIXED>
+__device__ __forceinline__ void point_taps_pre(const TiledArgs<T, CK>& a, const AxisGeom* ax, const Sub& s,
+                                               const T (&t)[3], int m, cx<T> ph, PointTaps<T, CK, JMAX>& pt) {
+    const InterpArgs<T, CK>& ia = a.ia;
+    pt.m = m;
+    pt.ph = ph;
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) {
+        int k0;
+        if constexpr (CK) k0 = axis_coefs<T, JMAX, FIXED>(t[d], ia.L, ax[d], ia.h[d], pt.cr[d], pt.ci[d]);
+        else k0 = axis_coefs<T, JMAX, FIXED>(t[d], ia.L, ax[d], ia.h[d], pt.cr[d]);
+        pt.l[d] = k0 - s.o[d];
+    }
+}
+
 // sum over the J^d taps of the staged tile, in the reference's order
 // (FUSED: with fused multiply-adds)
 template <typename T, int NDIM, bool CK, int JMAX, bool FIXED, bool FUSED = false>
@@ -655,10 +671,45 @@ __global__ __launch_bounds__(256) void interp_tiled_kernel(TiledArgs<T, CK> a) {
             for (int r = 0; r < ia.nreps; ++r) {
                 stage_tile<T, NDIM, CK>(a, s, ax, ia.grid + ((int64_t)s.frame * ia.nreps + r) * ia.kstride, tiles);
                 __syncthreads();
-                for (int i = s.start + tid; i < s.end; i += blockDim.x) {
-                    PointTaps<T, CK, JMAX> pt;
-                    point_taps<T, NDIM, CK, JMAX, FIXED>(a, ax, s, i, pt);
-                    ia.out[(int64_t)r * ia.M + pt.m] = gather_taps<T, NDIM, CK, JMAX, FIXED, FUSED>(tiles, pt, ax, T0, T1, a.phs);
+                if constexpr (NDIM == 3) {
+                    // several points per thread (3-D subproblems hold up to
+                    // 2,048): the next point's position, index and phase
+                    // load while this one's taps are summed (measured 8%
+                    // faster at Kd 256^3; occupancy is bound by the tile's
+                    // LDS there, not by the extra registers)
+                    int i = s.start + tid;
+                    T tn[3] = {0, 0, 0};
+                    int mn = 0;
+                    cx<T> pn = mkcx<T>((T)1, (T)0);
+                    if (i < s.end) {
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) tn[d] = a.tms[(int64_t)d * ia.M + i];
+                        mn = a.bs.perm[i];
+                        if (a.phs) pn = a.phs[i];
+                    }
+                    for (; i < s.end; i += blockDim.x) {
+                        const T tc[3] = {tn[0], tn[1], tn[2]};
+                        const int mc = mn;
+                        const cx<T> pc = pn;
+                        const int inext = i + blockDim.x;
+                        if (inext < s.end) {
+#pragma unroll
+                            for (int d = 0; d < 3; ++d) tn[d] = a.tms[(int64_t)d * ia.M + inext];
+                            mn = a.bs.perm[inext];
+                            if (a.phs) pn = a.phs[inext];
+                        }
+                        PointTaps<T, CK, JMAX> pt;
+                        point_taps_pre<T, NDIM, CK, JMAX, FIXED>(a, ax, s, tc, mc, pc, pt);
+                        ia.out[(int64_t)r * ia.M + pt.m] =
+                            gather_taps<T, NDIM, CK, JMAX, FIXED, FUSED>(tiles, pt, ax, T0, T1, a.phs);
+                    }
+                } else {
+                    for (int i = s.start + tid; i < s.end; i += blockDim.x) {
+                        PointTaps<T, CK, JMAX> pt;
+                        point_taps<T, NDIM, CK, JMAX, FIXED>(a, ax, s, i, pt);
+                        ia.out[(int64_t)r * ia.M + pt.m] =
+                            gather_taps<T, NDIM, CK, JMAX, FIXED, FUSED>(tiles, pt, ax, T0, T1, a.phs);
+                    }
                 }
                 __syncthreads();
             }
