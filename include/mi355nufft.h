@@ -156,8 +156,9 @@ int mi355nufft_set_tiling(mi355nufft_plan *plan, const int32_t *bin_size,
  * event pair on the plan's stream, timestamped by the kernel's dispatch
  * (hipExtLaunchKernel), so the pair spans the kernel's execution only.  mi355nufft_read_timing waits for
  * the recorded launches, returns their number and summed duration in
- * milliseconds, and resets both.  Instrumentation for benchmarks; no
- * counterpart in the reference.
+ * milliseconds, and resets both.  enable: 0 off, 1 events, 3 events and the
+ * scatter's flush atomic counts (mi355nufft_read_flush_atomics).
+ * Instrumentation for benchmarks; no counterpart in the reference.
  */
 int mi355nufft_set_timing(mi355nufft_plan *plan, int32_t enable);
 int mi355nufft_read_timing(mi355nufft_plan *plan, int32_t which, int64_t *launches,
