@@ -646,16 +646,20 @@ def main():
     peaks = measure_atomic_peaks()
     print("bench: atomic peaks (lane ops/s) %s" % json.dumps({k: "%.3g" % v for k, v in peaks.items()}),
           file=sys.stderr, flush=True)
-    # the C2 scatter accumulates in float64: M * J^d complex accumulations
-    # per launch, two ds_add_f64 lane operations each
+    # M * J^d complex accumulations per scatter launch, two LDS atomic lane
+    # operations each: ds_add_u32 with the fixed-point tile, else ds_add_f64
     acc = run.M * 6 ** 2
     t_sc = kern["spread"][1] * 1e-3
+    fxp = run.A.plan_info()["fixed_point_scatter"]
+    lds_peak = peaks["lds_u32"] if fxp else peaks["lds_f64"]
     roofline["scatter_lds_atomics"] = {
         "accumulations_per_s": round(acc / t_sc, 1),
         "lane_atomics_per_s": round(2 * acc / t_sc, 1),
-        "peak_lane_atomics_per_s": round(peaks["lds_f64"], 1),
-        "frac": round(2 * acc / t_sc / peaks["lds_f64"], 4),
-        "peak_basis": "ds_add_f64 at scattered addresses, measured here (mi355nufft_measure_atomics kind 0)",
+        "peak_lane_atomics_per_s": round(lds_peak, 1),
+        "frac": round(2 * acc / t_sc / lds_peak, 4),
+        "peak_basis": ("ds_add_u32" if fxp else "ds_add_f64") + " at scattered addresses, measured here "
+                      "(mi355nufft_measure_atomics kind %d)" % (1 if fxp else 0),
+        "accumulation": "fixed32" if fxp else "float64",
     }
     roofline["atomic_peaks_lane_ops_per_s"] = {k: round(v, 1) for k, v in peaks.items()}
     # the flush's global float atomics (counted in the kernel) against the

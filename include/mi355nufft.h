@@ -177,13 +177,14 @@ int mi355nufft_read_flush_atomics(mi355nufft_plan *plan, uint64_t *lane_ops);
 /*
  * Accumulation of the tiled scatter's LDS tile (no counterpart in the
  * reference, whose scatter accumulates in the operator's precision):
- *   accumulate  0 auto: 3-D complex64 plans accumulate in 32-bit fixed point
- *               (per subproblem a power-of-two scale chosen so that no cell
- *               sum can overflow; relative rounding of order P * 2^-30 per
- *               cell, independent of the order of the atomics; about 3e-6
- *               relative L2 on a Kd 256^3 kooshball grid), all others in
- *               float64.  1: float64.  2: 32-bit fixed point for every
- *               complex64 plan (complex128 stays float64).
+ *   accumulate  0 auto: 2-D and 3-D complex64 plans accumulate in 32-bit
+ *               fixed point (per subproblem a power-of-two scale chosen so
+ *               that no cell sum can overflow; relative rounding of order
+ *               P * 2^-31 per cell, independent of the order of the atomics;
+ *               about 8e-7 relative L2 from the float64 tile on a Kd 256^3
+ *               kooshball grid, 3e-7 on a Kd 512^2 golden-angle grid), all
+ *               others in float64.  1: float64.  2: 32-bit fixed point for
+ *               every complex64 plan (complex128 stays float64).
  * Re-sorts the current points if any (the tile size decides whether the
  * tiled kernels fit the LDS).
  */

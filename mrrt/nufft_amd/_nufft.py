@@ -422,7 +422,7 @@ class NufftBase(object):
 
     def set_scatter(self, accumulate="auto"):
         """Accumulation of the scatter's LDS tile: "auto" (32-bit fixed point
-        for 3-D complex64, float64 otherwise), "float64", or "fixed32"
+        for 2-D and 3-D complex64, float64 otherwise), "float64", or "fixed32"
         (32-bit fixed point for any complex64 plan; see
         mi355nufft_set_scatter)."""
         self._sync_stream()

@@ -137,9 +137,9 @@ struct mi355nufft_plan {
     void* d_tiles = nullptr;  // two-phase scatter tile slots
     size_t tiles_bytes = 0;
     void* tiles_target = nullptr;  // set while a two-phase scatter runs
-    // scatter accumulation: 0 auto (32-bit fixed point for 3-D complex64
-    // when the table bound is known, else float64), 1 float64, 2 32-bit
-    // fixed point for complex64
+    // scatter accumulation: 0 auto (32-bit fixed point for 2-D and 3-D
+    // complex64 when the table bound is known, else float64), 1 float64,
+    // 2 32-bit fixed point for complex64
     int scatter_acc = 0;
     // prod_d max |h_d| over the plan's tables (bound of every coefficient
     // product); 0 when unknown (raw entry points: caller tables)
@@ -198,13 +198,13 @@ namespace mi355host {
 constexpr int kFlushSlots = 1024;
 
 // Whether the scatter tile accumulates complex64 in 32-bit fixed point (see
-// spread_tiled_kernel): by default in 3-D, where it measured 26% faster at
-// Kd 256^3 for a relative error of about 3e-6 (2-D gained under 10% for
-// 7x the error), or on request; it needs the plan's table bound.  Else the
+// spread_tiled_kernel): by default in 2-D and 3-D, or on request; it needs
+// the plan's table bound.  Measured: Kd 256^3 scatter 26% faster (about
+// 8e-7 from the float64 tile), Kd 512^2 6% faster (about 3e-7).  Else the
 // tile accumulates in float64.
 inline bool fxp_eligible(const mi355nufft_plan* p) {
     if (p->precision != 0 || !(p->tab_cmax > 0.0 && p->tab_cmax < 1e30) || p->scatter_acc == 1) return false;
-    return p->scatter_acc == 2 || p->ndim == 3;
+    return p->scatter_acc == 2 || p->ndim >= 2;
 }
 inline bool use_fxp(const mi355nufft_plan* p) {
     // the int32 tile's bound holds up to 2^20 points per subproblem

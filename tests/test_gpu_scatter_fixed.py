@@ -57,9 +57,10 @@ def test_fixed_point_vs_oracle_and_float64(ndim, phasing):
 
 
 def test_default_accumulation_per_dimension():
-    """3-D complex64 plans default to the fixed-point tile, 2-D and
+    """2-D and 3-D complex64 plans default to the fixed-point tile,
     complex128 plans to float64."""
-    for ndim, precision, want in ((2, "single", False), (3, "single", True), (3, "double", False)):
+    for ndim, precision, want in ((2, "single", True), (3, "single", True), (2, "double", False),
+                                  (3, "double", False)):
         Nd, Kd, om = _case(ndim)
         A = NufftBase(Nd, om, Jd=6, Kd=Kd, Ld=512, precision=precision, device=dev())
         assert A.plan_info()["fixed_point_scatter"] == want, (ndim, precision)
