@@ -79,6 +79,26 @@ def test_2d_config_full_size_vs_oracle_and_nudft(config, precision):
     assert e_fn < 1e-3 and e_an < 1e-3
 
 
+def test_c2_full_size_adjoint_fixed_point_vs_float64():
+    """C2 adjoint of 205,824 random samples: the default 32-bit fixed-point
+    scatter tile against the float64 tile, and both against the oracle."""
+    wl = W.c2("single")
+    geo = _ops(wl, precision="single")
+    A = NufftBase(wl["Nd"], wl["om"], device=dev(), **geo)
+    B = O.NufftOracle(wl["Nd"], wl["om"], **geo)
+    rng = np.random.default_rng(16)
+    y = (rng.standard_normal(A.M) + 1j * rng.standard_normal(A.M)).astype(np.complex64)
+    assert A.plan_info()["fixed_point_scatter"]
+    xf = A.adj(y)
+    A.set_scatter("float64")
+    xd = A.adj(y)
+    xo = B.adj(y)
+    e = report("c2-c64", "adjoint-fixed-point-vs-float64-tile(full size)", rel_l2(xf, xd))
+    e_f = report("c2-c64", "adjoint-fixed-point-vs-oracle", rel_l2(xf, xo))
+    e_d = report("c2-c64", "adjoint-float64-tile-vs-oracle", rel_l2(xd, xo))
+    assert e <= 5e-6 and e_f <= 1e-5 and e_d <= 1e-5
+
+
 # ---------------------------------------------------------------------- C3
 @pytest.fixture(scope="module")
 def c3_setup():
