@@ -763,16 +763,17 @@ __device__ __forceinline__ cx<T> sample_times_conj_phase(const InterpArgs<T, CK>
 // REPS (several repetitions): when a subproblem has at most one point per
 // thread, each thread computes its point's coefficients once and keeps them
 // in registers for every repetition, instead of once per repetition.
-template <typename T, int NDIM, bool CK, int JMAX, bool FIXED, bool FXP, bool REPS = false>
+template <typename T, int NDIM, bool CK, int JMAX, bool FIXED, bool FXP, bool REPS = false, int RC = 1>
 __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
     static_assert(!FXP || sizeof(T) == 4, "fixed-point accumulation is for complex64");
+    static_assert(RC == 1 || !REPS, "the coefficient cache is for one repetition per pass");
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ float red[4];  // block_sum, at most 256 threads
     using Acc = typename std::conditional<FXP, int, double>::type;
     const InterpArgs<T, CK>& ia = a.ia;
     const BinGeom& bg = a.bg;
-    Acc* tre = reinterpret_cast<Acc*>(smem);
-    Acc* tim = tre + bg.tile;
+    // RC tiles, each its real then its imaginary plane
+    Acc* const tiles = reinterpret_cast<Acc*>(smem);
     const AxisGeom ax[3] = {ia.ax[0], ia.ax[1], ia.ax[2]};
     const int T0 = bg.T[0], T1 = bg.T[1];
     const int tid = threadIdx.x;
@@ -787,36 +788,57 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
         PointTaps<T, CK, JMAX> pc;
         if constexpr (REPS)
             if (cached && ic < s.end) point_taps<T, NDIM, CK, JMAX, FIXED>(a, ax, s, ic, pc);
-        for (int r = 0; r < ia.nreps; ++r) {
-            for (int e = tid; e < bg.tile; e += blockDim.x) {
-                tre[e] = 0;
-                tim[e] = 0;
+        // RC repetitions per pass, each with its own tile: one barrier round
+        // and one coefficient computation per point serve all of them
+        for (int r0 = 0; r0 < ia.nreps; r0 += RC) {
+            const int rc = min(RC, ia.nreps - r0);  // uniform over the workgroup
+            for (int e = tid; e < 2 * rc * bg.tile; e += blockDim.x) tiles[e] = 0;
+            int sh[RC];     // FXP: tap values of repetition r0 + k are scaled by 2^sh[k]
+            bool bad[RC];   // FXP: a non-finite sample
+#pragma unroll
+            for (int k = 0; k < RC; ++k) {
+                sh[k] = 0;
+                bad[k] = false;
             }
-            int sh = 0;        // FXP: tap values are scaled by 2^sh
-            bool bad = false;  // FXP: a non-finite sample
             if constexpr (FXP) {
-                float part = 0.f;
+                float part[RC];
+#pragma unroll
+                for (int k = 0; k < RC; ++k) part[k] = 0.f;
                 for (int i = s.start + tid; i < s.end; i += blockDim.x) {
                     const int64_t m = a.bs.perm[i];
-                    const cx<T> y = ia.in[(int64_t)r * ia.M + m];
-                    part += fabsf(y.x) + fabsf(y.y);  // |f| = |y| per component bound (|phase| = 1)
+#pragma unroll
+                    for (int k = 0; k < RC; ++k)
+                        if (k < rc) {
+                            const cx<T> y = ia.in[(int64_t)(r0 + k) * ia.M + m];
+                            part[k] += fabsf(y.x) + fabsf(y.y);  // |f| = |y| per component bound (|phase| = 1)
+                        }
                 }
-                const float A = block_sum(part, red) * a.cmax * 1.001f;
-                bad = !(A <= 3.0e38f);
-                int e = 0;
-                (void)frexpf(A, &e);
-                sh = 31 - e;
+#pragma unroll
+                for (int k = 0; k < RC; ++k)
+                    if (k < rc) {
+                        const float A = block_sum(part[k], red) * a.cmax * 1.001f;
+                        bad[k] = !(A <= 3.0e38f);
+                        int e = 0;
+                        (void)frexpf(A, &e);
+                        sh[k] = 31 - e;
+                    }
             }
             __syncthreads();
             auto add_point = [&](const PointTaps<T, CK, JMAX>& pt) {
-                cx<T> f = sample_times_conj_phase<T, CK, JMAX>(ia, r, pt, a.phs);
-                if constexpr (FXP) {
-                    if (bad) return;
-                    f.x = ldexpf(f.x, sh);
-                    f.y = ldexpf(f.y, sh);
-                    scatter_taps_fx<NDIM, CK, JMAX, FIXED>(tre, tim, pt, f, ax, T0, T1);
-                } else {
-                    scatter_taps<T, NDIM, CK, JMAX, FIXED>(tre, tim, pt, f, ax, T0, T1);
+#pragma unroll
+                for (int k = 0; k < RC; ++k) {
+                    if (k >= rc) break;
+                    Acc* tre = tiles + 2 * k * bg.tile;
+                    Acc* tim = tre + bg.tile;
+                    cx<T> f = sample_times_conj_phase<T, CK, JMAX>(ia, r0 + k, pt, a.phs);
+                    if constexpr (FXP) {
+                        if (bad[k]) continue;
+                        f.x = ldexpf(f.x, sh[k]);
+                        f.y = ldexpf(f.y, sh[k]);
+                        scatter_taps_fx<NDIM, CK, JMAX, FIXED>(tre, tim, pt, f, ax, T0, T1);
+                    } else {
+                        scatter_taps<T, NDIM, CK, JMAX, FIXED>(tre, tim, pt, f, ax, T0, T1);
+                    }
                 }
             };
             if (cached) {
@@ -829,64 +851,72 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
                 }
             }
             __syncthreads();
-            auto value = [&](int e, T& vr, T& vi) {  // false: an empty cell
-                if constexpr (FXP) {
-                    const int ir = tre[e], ii = tim[e];
-                    if (bad) {
-                        vr = vi = __int_as_float(0x7fc00000);
-                        return true;
+#pragma unroll
+            for (int k = 0; k < RC; ++k) {
+                if (k >= rc) break;
+                const int r = r0 + k;
+                const Acc* tre = tiles + 2 * k * bg.tile;
+                const Acc* tim = tre + bg.tile;
+                const int shk = sh[k];
+                const bool badk = bad[k];
+                auto value = [&](int e, T& vr, T& vi) {  // false: an empty cell
+                    if constexpr (FXP) {
+                        const int ir = tre[e], ii = tim[e];
+                        if (badk) {
+                            vr = vi = __int_as_float(0x7fc00000);
+                            return true;
+                        }
+                        if (ir == 0 && ii == 0) return false;
+                        vr = ldexpf((float)ir, -shk);
+                        vi = ldexpf((float)ii, -shk);
+                    } else {
+                        const double ar = tre[e], ai = tim[e];
+                        if (ar == 0.0 && ai == 0.0) return false;
+                        vr = (T)ar;
+                        vi = (T)ai;
                     }
-                    if (ir == 0 && ii == 0) return false;
-                    vr = ldexpf((float)ir, -sh);
-                    vi = ldexpf((float)ii, -sh);
-                } else {
-                    const double ar = tre[e], ai = tim[e];
-                    if (ar == 0.0 && ai == 0.0) return false;
-                    vr = (T)ar;
-                    vi = (T)ai;
+                    return true;
+                };
+                if (a.tiles) {  // two-phase: write the whole tile to its slot, no phase
+                    cx<T>* __restrict__ slot = a.tiles + ((int64_t)sid * ia.nreps + r) * bg.tile;
+                    for (int e = tid; e < bg.tile; e += blockDim.x) {
+                        T vr = 0, vi = 0;
+                        (void)value(e, vr, vi);
+                        slot[e] = mkcx<T>(vr, vi);
+                    }
+                    continue;
                 }
-                return true;
-            };
-            if (a.tiles) {  // two-phase: write the whole tile to its slot, no phase
-                cx<T>* __restrict__ slot = a.tiles + ((int64_t)sid * ia.nreps + r) * bg.tile;
+                cx<T>* __restrict__ g = ia.grid_out + ((int64_t)s.frame * ia.nreps + r) * ia.kstride;
+                // A bin's own cells past the previous bin's halo (local e_d in
+                // [J_d - 1, B_d) on every axis) lie in no other tile when every
+                // K_d is a multiple of B_d; when the bin has a single
+                // subproblem they are written with plain stores (the grid is
+                // zero there), everything else with atomics.
+                const bool own = a.own_ok && s.solo;
                 for (int e = tid; e < bg.tile; e += blockDim.x) {
-                    T vr = 0, vi = 0;
-                    (void)value(e, vr, vi);
-                    slot[e] = mkcx<T>(vr, vi);
-                }
-                __syncthreads();
-                continue;
-            }
-            cx<T>* __restrict__ g = ia.grid_out + ((int64_t)s.frame * ia.nreps + r) * ia.kstride;
-            // A bin's own cells past the previous bin's halo (local e_d in
-            // [J_d - 1, B_d) on every axis) lie in no other tile when every
-            // K_d is a multiple of B_d; when the bin has a single
-            // subproblem they are written with plain stores (the grid is
-            // zero there), everything else with atomics.
-            const bool own = a.own_ok && s.solo;
-            for (int e = tid; e < bg.tile; e += blockDim.x) {
-                T vr, vi;
-                if (!value(e, vr, vi)) continue;
-                int g0, g1, g2, e0, e1, e2;
-                const int64_t gi = tile_to_grid<NDIM>(e, bg, s, ax, g0, g1, g2, e0, e1, e2);
-                if (a.use_pb) {  // * conj(phase_before)
-                    const T c[3] = {a.pb[0], a.pb[1], a.pb[2]};
-                    const cx<T> p = cis_tile(phase_before_arg<T>(c, NDIM, g0, g1, g2));
-                    const T wr = vr * p.x + vi * p.y;
-                    const T wi = vi * p.x - vr * p.y;
-                    vr = wr;
-                    vi = wi;
-                }
-                bool mine = own && e0 >= ax[0].J - 1 && e0 < bg.B[0];
-                if (NDIM > 1) mine = mine && e1 >= ax[1].J - 1 && e1 < bg.B[1];
-                if (NDIM > 2) mine = mine && e2 >= ax[2].J - 1 && e2 < bg.B[2];
-                if (mine) {
-                    g[gi] = mkcx<T>(vr, vi);
-                } else {
-                    T* q = reinterpret_cast<T*>(g + gi);
-                    unsafeAtomicAdd(q, vr);
-                    unsafeAtomicAdd(q + 1, vi);
-                    natom += 2;
+                    T vr, vi;
+                    if (!value(e, vr, vi)) continue;
+                    int g0, g1, g2, e0, e1, e2;
+                    const int64_t gi = tile_to_grid<NDIM>(e, bg, s, ax, g0, g1, g2, e0, e1, e2);
+                    if (a.use_pb) {  // * conj(phase_before)
+                        const T c[3] = {a.pb[0], a.pb[1], a.pb[2]};
+                        const cx<T> p = cis_tile(phase_before_arg<T>(c, NDIM, g0, g1, g2));
+                        const T wr = vr * p.x + vi * p.y;
+                        const T wi = vi * p.x - vr * p.y;
+                        vr = wr;
+                        vi = wi;
+                    }
+                    bool mine = own && e0 >= ax[0].J - 1 && e0 < bg.B[0];
+                    if (NDIM > 1) mine = mine && e1 >= ax[1].J - 1 && e1 < bg.B[1];
+                    if (NDIM > 2) mine = mine && e2 >= ax[2].J - 1 && e2 < bg.B[2];
+                    if (mine) {
+                        g[gi] = mkcx<T>(vr, vi);
+                    } else {
+                        T* q = reinterpret_cast<T*>(g + gi);
+                        unsafeAtomicAdd(q, vr);
+                        unsafeAtomicAdd(q + 1, vi);
+                        natom += 2;
+                    }
                 }
             }
             __syncthreads();

@@ -17,12 +17,16 @@ void launch_tiled(bool adjoint, const TiledArgs<T, CK>& a, unsigned nsub, unsign
     // one point per thread (the cache costs registers even when unused:
     // 133 VGPRs against 89 for the 2-D J = 6 scatter)
     const bool reps = a.ia.nreps > 1 && a.bg.P <= (int)threads;
-    auto spread = reps ? spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED, false, true>
-                       : spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED, false, false>;
+    // a.rc == 2 for the scatter: two repetitions' tiles per pass
+    const bool pair = adjoint && a.rc == 2;
+    auto spread = pair ? spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED, false, false, 2>
+                       : reps ? spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED, false, true>
+                              : spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED, false, false>;
     if constexpr (sizeof(T) == 4)
         if (a.cmax > 0.f)
-            spread = reps ? spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED, true, true>
-                          : spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED, true, false>;
+            spread = pair ? spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED, true, false, 2>
+                          : reps ? spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED, true, true>
+                                 : spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED, true, false>;
     auto interp = a.rc > 1 ? interp_tiled_kernel<T, NDIM, CK, JMAX, FIXED, true>
                            : interp_tiled_kernel<T, NDIM, CK, JMAX, FIXED, false>;
     if constexpr (FIXED)  // fused tap sums for the J = 6 kernels unless the reference's roundings are asked for
@@ -71,13 +75,16 @@ int run_tiled(mi355nufft_plan* p, bool adjoint, const void* samples_in, void* sa
     const unsigned nsub = tiled_grid(p);
     // the gather stages up to kMaxRepChunk repetitions' tiles at once
     // (coefficients are then computed once per point for all of them),
-    // within the LDS budget; the scatter one at a time
+    // within the LDS budget; the scatter one or two at a time
     const size_t one = adjoint ? tile_bytes(p) : gather_tile_bytes(p);
     const size_t chunk_lds = p->lds_limit < kRepChunkLds ? p->lds_limit : kRepChunkLds;
     int rc = adjoint ? 1 : (int)(chunk_lds / one);
     rc = rc < kMaxRepChunk ? rc : kMaxRepChunk;
     rc = rc < nreps ? rc : nreps;
     a.rc = rc > 1 ? rc : 1;
+    // the scatter takes two repetitions per pass when both tiles fit the
+    // chunk budget and its points would not fit the coefficient cache
+    if (adjoint && nreps >= 2 && kScatterPairs && 2 * one <= chunk_lds && p->bg.P > p->bg_threads) a.rc = 2;
     const size_t lds = one * (size_t)a.rc;
     const int jm = jmax_of(p->ndim, J);
     const int jf = fixed_j(p->ndim, J);
