@@ -106,3 +106,29 @@ def test_fixed_point_range_nan_and_zero():
     yn = y.copy()
     yn[17] = np.nan
     assert np.isnan(A.adj(yn)).any()
+
+
+@pytest.mark.parametrize("method", ["auto", "tiled_two_phase"])
+def test_two_repetitions_per_pass_vs_oracle(method):
+    """Subproblems larger than the workgroup (512 points, 256 threads) with
+    several repetitions: the scatter accumulates two repetitions per pass
+    (two fixed-point tiles of 32x32 bins fit the chunk budget), and an odd
+    count leaves a last pass of one.  Each repetition has its own scale:
+    the second is 2^40 times the first."""
+    Nd, Kd = (64, 64), (128, 128)
+    om = trj.golden_angle_2d(60, 128)
+    A = NufftBase(Nd, om, Jd=6, Kd=Kd, Ld=1024, device=dev())
+    B = O.NufftOracle(Nd, om, Jd=6, Kd=Kd, Ld=1024)
+    A.set_binning((32, 32), 512, method)
+    assert A.plan_info()["fixed_point_scatter"]
+    rng = np.random.default_rng(21)
+    y = (rng.standard_normal((om.shape[0], 3)) + 1j * rng.standard_normal((om.shape[0], 3))).astype(np.complex64)
+    y[:, 1] *= np.float32(2.0 ** 40)
+    x = A.adj(y)
+    xo = B.adj(y)
+    for r in range(3):
+        assert rel_l2(x[..., r], xo[..., r]) <= 1e-5, r
+    A.set_scatter("float64")
+    xd = A.adj(y)
+    for r in range(3):
+        assert rel_l2(x[..., r].astype(np.complex128), xd[..., r].astype(np.complex128)) <= 5e-6, r
