@@ -741,8 +741,10 @@ __device__ __forceinline__ cx<T> sample_times_conj_phase(const InterpArgs<T, CK>
     return mkcx<T>(f.x * pt.ph.x + f.y * pt.ph.y, f.y * pt.ph.x - f.x * pt.ph.y);
 }
 
-// One repetition at a time: staging several repetitions' tiles was measured
-// no faster (the LDS atomics dominate) and costs occupancy.
+// RC repetitions per pass, each with its own tile (RC = 2 when both tiles
+// fit the LDS budget and a subproblem has more points than threads, so
+// that each point's coefficients serve two repetitions; more tiles per pass
+// were measured no faster, the LDS atomics dominating, and cost occupancy).
 //
 // FXP (complex64 only): the tile accumulates in 32-bit fixed point.  On
 // gfx950 an LDS float64 atomic add at scattered addresses costs about 21

@@ -698,8 +698,10 @@ int fft_impl(mi355nufft_plan* p, void* grid, int32_t nreps, int direction) {
 namespace mi355host {
 
 // Default bin geometry (autotuned on golden-angle radial and kooshball
-// trajectories): 2-D 16x16 with P = 256 points per subproblem, 3-D 16x8x8
-// with P = 1024, 1-D 512 with P = 512.  Requested sizes override.
+// trajectories): 2-D 16x16 with P = 256 points per subproblem; 3-D 16^3
+// with P = 2048 when the scatter tile is fixed point and no sparse matrix is
+// set, else 16x8x8 with P = 1024; 1-D 512 with P = 512.  Requested sizes
+// override.
 void default_bins(const mi355nufft_plan* p, int B[3], int& P) {
     for (int d = 0; d < 3; ++d) B[d] = 1;
     if (p->ndim == 1) {
@@ -775,10 +777,6 @@ void setup_bin_geom(mi355nufft_plan* p) {
 
 namespace {
 
-// LDS of a tiled launch: the gather stages complex T, the scatter
-// accumulates complex float64 (see spread_tiled_kernel); the larger of the two
-// must fit the budget since both share the binning.
-
 template <typename Tp>
 int grow(mi355nufft_plan* p, Tp** buf, int64_t& cap, int64_t need, int64_t per) {
     if (need <= cap && *buf) return MI355NUFFT_OK;
@@ -801,6 +799,8 @@ int bin_points_impl(mi355nufft_plan* p) {
     setup_bin_geom(p);
     if (p->method == 4 && !p->reduce_ok)
         return fail(MI355NUFFT_EINVAL, "bins too small for the two-phase scatter");
+    // the scatter's tile (complex float64 or int32 pairs) is never smaller
+    // than the gather's (complex T), and both kernels share the binning
     if (tile_bytes(p) > p->lds_limit) {
         if (p->method >= 2 || p->nframes > 1)
             return fail(MI355NUFFT_EINVAL, "tile of %d points (%zu bytes) exceeds the LDS budget of %zu bytes",
