@@ -29,7 +29,7 @@ sys.path.insert(0, HERE)
 HBM_PEAK_GBS = 8000.0
 
 KERNEL_NAMES = {"interp": "interp_tiled_kernel<float, 2, false, 6, true, false, true>",
-                "spread": "spread_tiled_kernel<float, 2, false, 6, true, false>"}
+                "spread": "spread_tiled_kernel<float, 2, false, 6, true, true, true>"}
 
 # HBM bytes per launch of the 2-D 256^2 complex64 kernels, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE passes over this bench (--calibrate-copy), each
@@ -40,7 +40,7 @@ TRAFFIC_BYTES = {"spread": 20259142, "interp": 13947921}
 # scatter tile; profiles/nufft3d_128_c64_hbm_traffic.md) and the 24-frame x
 # 32-coil batch (profiles/nufft2d_256_c64_frames_kernel_stats_and_traffic.md).
 TRAFFIC_BYTES_C3 = {"interp": 385.8e6, "spread": 546.6e6}
-TRAFFIC_BYTES_C4 = {"interp": 2467.7e6, "spread": 2035.8e6}  # 64x32 bins, 1,024 points per subproblem
+TRAFFIC_BYTES_C4 = {"interp": 2468.2e6, "spread": 2102.3e6}  # 64x32 bins, 1,024 points per subproblem
 
 
 def parse():

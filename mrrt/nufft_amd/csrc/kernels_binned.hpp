@@ -734,11 +734,15 @@ __global__ __launch_bounds__(256) void interp_tiled_kernel(TiledArgs<T, CK> a) {
 }
 
 template <typename T, bool CK, int JMAX>
-__device__ __forceinline__ cx<T> sample_times_conj_phase(const InterpArgs<T, CK>& ia, int r,
-                                                         const PointTaps<T, CK, JMAX>& pt, bool has_ph) {
-    const cx<T> f = ia.in[(int64_t)r * ia.M + pt.m];
+__device__ __forceinline__ cx<T> times_conj_phase(cx<T> f, const PointTaps<T, CK, JMAX>& pt, bool has_ph) {
     if (!has_ph) return f;
     return mkcx<T>(f.x * pt.ph.x + f.y * pt.ph.y, f.y * pt.ph.x - f.x * pt.ph.y);
+}
+
+template <typename T, bool CK, int JMAX>
+__device__ __forceinline__ cx<T> sample_times_conj_phase(const InterpArgs<T, CK>& ia, int r,
+                                                         const PointTaps<T, CK, JMAX>& pt, bool has_ph) {
+    return times_conj_phase<T, CK, JMAX>(ia.in[(int64_t)r * ia.M + pt.m], pt, has_ph);
 }
 
 // RC repetitions per pass, each with its own tile (RC = 2 when both tiles
@@ -762,9 +766,13 @@ __device__ __forceinline__ cx<T> sample_times_conj_phase(const InterpArgs<T, CK>
 // Non-finite samples make the whole tile NaN.  Without FXP (and for
 // complex128) the tile accumulates in float64.
 //
-// REPS (several repetitions): when a subproblem has at most one point per
-// thread, each thread computes its point's coefficients once and keeps them
-// in registers for every repetition, instead of once per repetition.
+// REPS (subproblems of at most one point per thread): each thread computes
+// its point's coefficients once, before the first repetition, and keeps them
+// in registers for every repetition.  With the fixed-point tile the sample
+// that the scale's block sum reads is also kept for the scatter, so the
+// coefficient loads run beside the sample reads and no load follows the
+// sum's barriers.  Taken for several repetitions, and for one with the
+// fixed-point tile.
 template <typename T, int NDIM, bool CK, int JMAX, bool FIXED, bool FXP, bool REPS = false, int RC = 1>
 __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
     static_assert(!FXP || sizeof(T) == 4, "fixed-point accumulation is for complex64");
@@ -802,18 +810,26 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
                 sh[k] = 0;
                 bad[k] = false;
             }
+            cx<T> yc = mkcx<T>((T)0, (T)0);  // REPS, fixed point: this thread's sample of repetition r0
             if constexpr (FXP) {
                 float part[RC];
 #pragma unroll
                 for (int k = 0; k < RC; ++k) part[k] = 0.f;
-                for (int i = s.start + tid; i < s.end; i += blockDim.x) {
-                    const int64_t m = a.bs.perm[i];
+                if (cached) {
+                    if (ic < s.end) {
+                        yc = ia.in[(int64_t)r0 * ia.M + pc.m];
+                        part[0] = fabsf(yc.x) + fabsf(yc.y);
+                    }
+                } else {
+                    for (int i = s.start + tid; i < s.end; i += blockDim.x) {
+                        const int64_t m = a.bs.perm[i];
 #pragma unroll
-                    for (int k = 0; k < RC; ++k)
-                        if (k < rc) {
-                            const cx<T> y = ia.in[(int64_t)(r0 + k) * ia.M + m];
-                            part[k] += fabsf(y.x) + fabsf(y.y);  // |f| = |y| per component bound (|phase| = 1)
-                        }
+                        for (int k = 0; k < RC; ++k)
+                            if (k < rc) {
+                                const cx<T> y = ia.in[(int64_t)(r0 + k) * ia.M + m];
+                                part[k] += fabsf(y.x) + fabsf(y.y);  // |f| = |y| per component bound (|phase| = 1)
+                            }
+                    }
                 }
 #pragma unroll
                 for (int k = 0; k < RC; ++k)
@@ -826,25 +842,31 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
                     }
             }
             __syncthreads();
+            // sample f (times conj(phase)) of repetition r0 + k at point pt
+            auto add_sample = [&](int k, const PointTaps<T, CK, JMAX>& pt, cx<T> f) {
+                Acc* tre = tiles + 2 * k * bg.tile;
+                Acc* tim = tre + bg.tile;
+                if constexpr (FXP) {
+                    if (bad[k]) return;
+                    f.x = ldexpf(f.x, sh[k]);
+                    f.y = ldexpf(f.y, sh[k]);
+                    scatter_taps_fx<NDIM, CK, JMAX, FIXED>(tre, tim, pt, f, ax, T0, T1);
+                } else {
+                    scatter_taps<T, NDIM, CK, JMAX, FIXED>(tre, tim, pt, f, ax, T0, T1);
+                }
+            };
             auto add_point = [&](const PointTaps<T, CK, JMAX>& pt) {
 #pragma unroll
                 for (int k = 0; k < RC; ++k) {
                     if (k >= rc) break;
-                    Acc* tre = tiles + 2 * k * bg.tile;
-                    Acc* tim = tre + bg.tile;
-                    cx<T> f = sample_times_conj_phase<T, CK, JMAX>(ia, r0 + k, pt, a.phs);
-                    if constexpr (FXP) {
-                        if (bad[k]) continue;
-                        f.x = ldexpf(f.x, sh[k]);
-                        f.y = ldexpf(f.y, sh[k]);
-                        scatter_taps_fx<NDIM, CK, JMAX, FIXED>(tre, tim, pt, f, ax, T0, T1);
-                    } else {
-                        scatter_taps<T, NDIM, CK, JMAX, FIXED>(tre, tim, pt, f, ax, T0, T1);
-                    }
+                    add_sample(k, pt, sample_times_conj_phase<T, CK, JMAX>(ia, r0 + k, pt, a.phs));
                 }
             };
             if (cached) {
-                if (ic < s.end) add_point(pc);
+                if (ic < s.end) {
+                    if constexpr (FXP) add_sample(0, pc, times_conj_phase<T, CK, JMAX>(yc, pc, a.phs));
+                    else add_point(pc);
+                }
             } else {
                 for (int i = s.start + tid; i < s.end; i += blockDim.x) {
                     PointTaps<T, CK, JMAX> pt;

@@ -13,10 +13,11 @@ namespace mi355host {
 template <typename T, bool CK, int NDIM, int JMAX, bool FIXED>
 void launch_tiled(bool adjoint, const TiledArgs<T, CK>& a, unsigned nsub, unsigned threads, size_t lds,
                   hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
-    // coefficients cached across repetitions: only when a subproblem fits
+    // coefficients cached across repetitions (and, with the fixed-point
+    // tile, the sample read by the scale's sum): only when a subproblem fits
     // one point per thread (the cache costs registers even when unused:
-    // 133 VGPRs against 89 for the 2-D J = 6 scatter)
-    const bool reps = a.ia.nreps > 1 && a.bg.P <= (int)threads;
+    // 135 VGPRs against 110 for the 2-D J = 6 fixed-point scatter)
+    const bool reps = (a.ia.nreps > 1 || a.cmax > 0.f) && a.bg.P <= (int)threads;
     // a.rc == 2 for the scatter: two repetitions' tiles per pass
     const bool pair = adjoint && a.rc == 2;
     auto spread = pair ? spread_tiled_kernel<T, NDIM, CK, JMAX, FIXED, false, false, 2>
