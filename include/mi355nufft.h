@@ -125,11 +125,13 @@ int64_t mi355nufft_num_points(const mi355nufft_plan *plan);
  * Bin geometry and kernel choice for the interpolation stages (no counterpart
  * in the reference, whose kernels run one thread per sample unsorted).
  *   bin_size    ndim bin sizes in grid points (NULL or 0 entries: defaults
- *               2-D 16x16, 3-D 16x8x8, 1-D 512; the scatter tile, 8 bytes
+ *               2-D 16x16; 3-D 16x16x16 for complex64 with the fixed-point
+ *               scatter tile and no sparse matrix set, else 16x8x8; 1-D
+ *               512; the scatter tile, 8 bytes
  *               per point in fixed point or 16 in float64 (see
  *               mi355nufft_set_scatter), must fit the LDS budget)
- *   max_points  points per subproblem (0: default 256 in 2-D, 1024 in 3-D,
- *               512 in 1-D)
+ *   max_points  points per subproblem (0: default 384 in 2-D; 2048 in 3-D
+ *               with 16x16x16 bins, else 1024; 512 in 1-D)
  *   method      0 auto (tiled when the tile fits in LDS), 1 global-memory
  *               kernels on unsorted points, 2 tiled (error if it cannot),
  *               3 tiled with the scatter flushed by global atomics,

@@ -132,3 +132,39 @@ def test_two_repetitions_per_pass_vs_oracle(method):
     xd = A.adj(y)
     for r in range(3):
         assert rel_l2(x[..., r].astype(np.complex128), xd[..., r].astype(np.complex128)) <= 5e-6, r
+
+
+@pytest.mark.parametrize("nreps", [1, 3])
+@pytest.mark.parametrize("phasing", ["real", "complex"])
+def test_one_point_per_thread_cache_vs_oracle(nreps, phasing):
+    """Subproblems of at most one point per thread (P = threads = 256): the
+    scatter computes each point's coefficients once, before the scale's
+    block sum, and keeps the sample that sum read for the scatter, for one
+    repetition and for several (each with its own scale: the second is
+    2^40 times the first).  A NaN sample still makes its tile NaN."""
+    Nd, Kd, om = _case(2)
+    shift = np.asarray(Nd) / 2
+    A = NufftBase(Nd, om, Jd=6, Kd=Kd, Ld=512, phasing=phasing, n_shift=shift, device=dev())
+    B = O.NufftOracle(Nd, om, Jd=6, Kd=Kd, Ld=512, phasing=phasing, n_shift=shift)
+    A.set_binning((16, 16), 256, "tiled", 256)
+    info = A.plan_info()
+    assert info["fixed_point_scatter"] and info["max_points"] == 256 and info["threads"] == 256
+    rng = np.random.default_rng(30 + nreps)
+    shape = (om.shape[0],) if nreps == 1 else (om.shape[0], nreps)
+    y = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    if nreps > 1:
+        y[:, 1] *= np.float32(2.0 ** 40)
+    x = A.adj(y)
+    xo = B.adj(y)
+    xs, xos = (x[..., None], xo[..., None]) if nreps == 1 else (x, xo)
+    for r in range(nreps):
+        assert rel_l2(xs[..., r], xos[..., r]) <= 1e-5, r
+    A.set_scatter("float64")
+    xd = A.adj(y)
+    xds = xd[..., None] if nreps == 1 else xd
+    for r in range(nreps):
+        assert rel_l2(xds[..., r], xos[..., r]) <= 1e-5, r
+    A.set_scatter("fixed32")
+    yn = y.copy()
+    yn[17] = np.nan
+    assert np.isnan(A.adj(yn)).any()
