@@ -35,7 +35,7 @@ KERNEL_NAMES = {"interp": "interp_tiled_kernel<float, 2, false, 6, true, false, 
 # FETCH_SIZE and WRITE_SIZE passes over this bench (--calibrate-copy), each
 # scaled by what the counter reads for a 256 MiB device copy (FETCH_SIZE
 # x 2.0, WRITE_SIZE x 1.0); profiles/nufft2d_256_c64_hbm_traffic.md.
-TRAFFIC_BYTES = {"spread": 20259142, "interp": 13947921}
+TRAFFIC_BYTES = {"spread": 17916241, "interp": 12873432}
 # The same for the 3-D 128^3 / 256^3 kooshball kernels (16^3 bins, fixed-point
 # scatter tile; profiles/nufft3d_128_c64_hbm_traffic.md) and the 24-frame x
 # 32-coil batch (profiles/nufft2d_256_c64_frames_kernel_stats_and_traffic.md).
