@@ -798,6 +798,12 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
         PointTaps<T, CK, JMAX> pc;
         if constexpr (REPS)
             if (cached && ic < s.end) point_taps<T, NDIM, CK, JMAX, FIXED>(a, ax, s, ic, pc);
+        // KEEP (1-D and 2-D, fixed point, one repetition per pass, no cache):
+        // with at most two points per thread, the indices and samples that
+        // the scale's sum reads are kept in registers for the scatter
+        constexpr bool KEEP = FXP && !REPS && RC == 1 && NDIM < 3;
+        const bool kept = KEEP && s.end - s.start <= 2 * (int)blockDim.x;
+        const int i1 = ic + blockDim.x;
         // RC repetitions per pass, each with its own tile: one barrier round
         // and one coefficient computation per point serve all of them
         for (int r0 = 0; r0 < ia.nreps; r0 += RC) {
@@ -811,6 +817,8 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
                 bad[k] = false;
             }
             cx<T> yc = mkcx<T>((T)0, (T)0);  // REPS, fixed point: this thread's sample of repetition r0
+            int km0 = 0, km1 = 0;            // KEEP: indices and samples of points ic and i1
+            cx<T> ky0 = yc, ky1 = yc;
             if constexpr (FXP) {
                 float part[RC];
 #pragma unroll
@@ -819,6 +827,17 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
                     if (ic < s.end) {
                         yc = ia.in[(int64_t)r0 * ia.M + pc.m];
                         part[0] = fabsf(yc.x) + fabsf(yc.y);
+                    }
+                } else if (kept) {
+                    if (ic < s.end) {
+                        km0 = a.bs.perm[ic];
+                        ky0 = ia.in[(int64_t)r0 * ia.M + km0];
+                        part[0] = fabsf(ky0.x) + fabsf(ky0.y);
+                    }
+                    if (i1 < s.end) {
+                        km1 = a.bs.perm[i1];
+                        ky1 = ia.in[(int64_t)r0 * ia.M + km1];
+                        part[0] += fabsf(ky1.x) + fabsf(ky1.y);
                     }
                 } else {
                     for (int i = s.start + tid; i < s.end; i += blockDim.x) {
@@ -866,6 +885,20 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
                 if (ic < s.end) {
                     if constexpr (FXP) add_sample(0, pc, times_conj_phase<T, CK, JMAX>(yc, pc, a.phs));
                     else add_point(pc);
+                }
+            } else if (kept) {
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int i = u ? i1 : ic;
+                    if (i < s.end) {
+                        T t[3];
+#pragma unroll
+                        for (int d = 0; d < NDIM; ++d) t[d] = a.tms[(int64_t)d * ia.M + i];
+                        const cx<T> ph = a.phs ? a.phs[i] : mkcx<T>((T)1, (T)0);
+                        PointTaps<T, CK, JMAX> pt;
+                        point_taps_pre<T, NDIM, CK, JMAX, FIXED>(a, ax, s, t, u ? km1 : km0, ph, pt);
+                        add_sample(0, pt, times_conj_phase<T, CK, JMAX>(u ? ky1 : ky0, pt, a.phs));
+                    }
                 }
             } else {
                 for (int i = s.start + tid; i < s.end; i += blockDim.x) {

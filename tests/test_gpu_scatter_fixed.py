@@ -134,21 +134,25 @@ def test_two_repetitions_per_pass_vs_oracle(method):
         assert rel_l2(x[..., r].astype(np.complex128), xd[..., r].astype(np.complex128)) <= 5e-6, r
 
 
+@pytest.mark.parametrize("max_points,threads", [(256, 256), (512, 256), (200, 128)])
 @pytest.mark.parametrize("nreps", [1, 3])
 @pytest.mark.parametrize("phasing", ["real", "complex"])
-def test_one_point_per_thread_cache_vs_oracle(nreps, phasing):
+def test_points_kept_across_the_scale_sum_vs_oracle(max_points, threads, nreps, phasing):
     """Subproblems of at most one point per thread (P = threads = 256): the
     scatter computes each point's coefficients once, before the scale's
     block sum, and keeps the sample that sum read for the scatter, for one
     repetition and for several (each with its own scale: the second is
-    2^40 times the first).  A NaN sample still makes its tile NaN."""
+    2^40 times the first).  At most two points per thread (P 512 / 256
+    threads, P 200 / 128 threads) and one repetition: the indices and
+    samples the sum read are kept for the scatter (several repetitions take
+    the two-per-pass kernel).  A NaN sample still makes its tile NaN."""
     Nd, Kd, om = _case(2)
     shift = np.asarray(Nd) / 2
     A = NufftBase(Nd, om, Jd=6, Kd=Kd, Ld=512, phasing=phasing, n_shift=shift, device=dev())
     B = O.NufftOracle(Nd, om, Jd=6, Kd=Kd, Ld=512, phasing=phasing, n_shift=shift)
-    A.set_binning((16, 16), 256, "tiled", 256)
+    A.set_binning((16, 16), max_points, "tiled", threads)
     info = A.plan_info()
-    assert info["fixed_point_scatter"] and info["max_points"] == 256 and info["threads"] == 256
+    assert info["fixed_point_scatter"] and info["max_points"] == max_points and info["threads"] == threads
     rng = np.random.default_rng(30 + nreps)
     shape = (om.shape[0],) if nreps == 1 else (om.shape[0], nreps)
     y = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
