@@ -54,9 +54,10 @@ struct BinState {
 // Points inside a bin are ordered so that consecutive points start their
 // footprints on different LDS banks.  The bank key of a point is the tile
 // offset of its first tap modulo kBanks (in 8-byte tile entries); the order
-// is by (rank inside the key, key), so any kBanks consecutive points of one
-// rank level have distinct keys and points in the same grid cell are at
-// least one level apart.  The same order serves the gather (tile reads) and
+// is by (rank inside the key, key, rotated by the rank in bins with an even
+// number of subproblems), so any kBanks consecutive points of one rank level
+// have distinct keys and points in the same grid cell are at least one level
+// apart.  The same order serves the gather (tile reads) and
 // the scatter (tile atomics).
 constexpr int kBanks = 32;
 
@@ -173,11 +174,15 @@ static __global__ __launch_bounds__(1024) void bin_scan_kernel(BinState bs, int 
 }
 
 // Sorted position of point m.  Its place in the bin's (rank, key) order is
-// pos = #{(k', r') : r' < r} + #{k' < key : count(k') > r}.  A bin of c points
-// has ns = ceil(c / P) subproblems, dealt round-robin over that order:
+// pos = #{(k', r') : r' < r} + #{k' before key : count(k') > r}.  A bin of c
+// points has ns = ceil(c / P) subproblems, dealt round-robin over that order:
 // subproblem j holds positions j, j + ns, j + 2 ns, ... (q or q + 1 points,
 // c = q ns + rem), so every subproblem samples all rank levels and the dense
-// cells of a bin do not pile up in its last subproblem.
+// cells of a bin do not pile up in its last subproblem.  Within a rank level
+// the keys run in order, rotated by the rank when ns is even: with full
+// levels a stride of ns would otherwise meet only 32 / gcd(ns, 32) keys
+// (a subproblem of a bin with ns = 16 sees two, and its lanes queue on two
+// LDS banks); rotated, consecutive levels supply the other residues.
 static __global__ __launch_bounds__(256) void bin_scatter_kernel(int64_t M, BinState bs, int P, SortGate gate) {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= M || gate_closed(gate)) return;
@@ -185,20 +190,22 @@ static __global__ __launch_bounds__(256) void bin_scatter_kernel(int64_t M, BinS
     const int bin = slot / kBanks;
     const int key = slot - bin * kBanks;
     const int r = bs.rank[m];
+    const int c = bs.counts[bin];
+    const int ns = (c + P - 1) / P;
+    const int rot = (ns & 1) ? 0 : r;  // key k comes at place (k - rot) mod kBanks of level r
+    const int kr = (key - rot) & (kBanks - 1);
     const int4* row = reinterpret_cast<const int4*>(bs.bkt + (int64_t)bin * kBanks);
     int pos = 0;
 #pragma unroll
     for (int k4 = 0; k4 < kBanks / 4; ++k4) {
         const int4 v = row[k4];
-        const int c[4] = {v.x, v.y, v.z, v.w};
+        const int cn[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int k = 4 * k4 + u;
-            pos += min(c[u], r) + ((k < key && c[u] > r) ? 1 : 0);
+            pos += min(cn[u], r) + ((((k - rot) & (kBanks - 1)) < kr && cn[u] > r) ? 1 : 0);
         }
     }
-    const int c = bs.counts[bin];
-    const int ns = (c + P - 1) / P;
     const int q = c / ns, rem = c - q * ns;
     const int j = pos % ns;
     bs.perm[bs.bin_start[bin] + j * q + min(j, rem) + pos / ns] = (int)m;
