@@ -35,11 +35,11 @@ KERNEL_NAMES = {"interp": "interp_tiled_kernel<float, 2, false, 6, true, false, 
 # FETCH_SIZE and WRITE_SIZE passes over this bench (--calibrate-copy), each
 # scaled by what the counter reads for a 256 MiB device copy (FETCH_SIZE
 # x 2.0, WRITE_SIZE x 1.0); profiles/nufft2d_256_c64_hbm_traffic.md.
-TRAFFIC_BYTES = {"spread": 17916241, "interp": 12873432}
+TRAFFIC_BYTES = {"spread": 18007664, "interp": 12857829}
 # The same for the 3-D 128^3 / 256^3 kooshball kernels (16^3 bins, fixed-point
 # scatter tile; profiles/nufft3d_128_c64_hbm_traffic.md) and the 24-frame x
 # 32-coil batch (profiles/nufft2d_256_c64_frames_kernel_stats_and_traffic.md).
-TRAFFIC_BYTES_C3 = {"interp": 385.8e6, "spread": 546.6e6}
+TRAFFIC_BYTES_C3 = {"interp": 375.8e6, "spread": 526.2e6}
 TRAFFIC_BYTES_C4 = {"interp": 2468.2e6, "spread": 2102.3e6}  # 64x32 bins, 1,024 points per subproblem
 
 
