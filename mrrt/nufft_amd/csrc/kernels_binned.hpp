@@ -678,12 +678,14 @@ __global__ __launch_bounds__(256) void interp_tiled_kernel(TiledArgs<T, CK> a) {
             for (int r = 0; r < ia.nreps; ++r) {
                 stage_tile<T, NDIM, CK>(a, s, ax, ia.grid + ((int64_t)s.frame * ia.nreps + r) * ia.kstride, tiles);
                 __syncthreads();
-                if constexpr (NDIM == 3) {
+                if constexpr (NDIM >= 2) {
                     // several points per thread (3-D subproblems hold up to
-                    // 2,048): the next point's position, index and phase
-                    // load while this one's taps are summed (measured 8%
-                    // faster at Kd 256^3; occupancy is bound by the tile's
-                    // LDS there, not by the extra registers)
+                    // 2,560, 2-D ones 384): the next point's position, index
+                    // and phase load while this one's taps are summed
+                    // (measured 8% faster at Kd 256^3, where the tile's LDS
+                    // bounds occupancy, not the extra registers; 2-D at Kd
+                    // 512^2 11.4 -> 11.1 us, 99 VGPRs, 4 waves per SIMD,
+                    // enough for its 1,001 subproblems)
                     int i = s.start + tid;
                     T tn[3] = {0, 0, 0};
                     int mn = 0;
