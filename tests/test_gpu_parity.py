@@ -421,6 +421,31 @@ def test_binning_variants_vs_oracle(method, bin_size, max_points, precision, pha
     assert rel_l2(A.adj(y), B.adj(y)) <= TOL[precision]
 
 
+@pytest.mark.parametrize("max_points", [500, 250, 125, 55, 54, 18])
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_dense_bin_dealt_subproblems_vs_oracle(max_points, precision):
+    """One 16x16 bin holding about 1,000 points, many of them in the same few
+    grid cells (deep rank levels), cut into ns = 2, 4, 8, 16, 17 and 48
+    subproblems (for these seeds): even counts take the rank-rotated key
+    order of the bin sort, odd ones the plain order.  Forward and adjoint against the oracle, with
+    the fixed-point and the float64 scatter tile for complex64."""
+    Nd, Kd = (32, 32), (64, 64)
+    rng = np.random.default_rng(max_points)
+    t = np.concatenate([rng.uniform(20.0, 36.0, (600, 2)),                      # spread over the bin
+                        np.repeat(rng.uniform(24.0, 30.0, (8, 2)), 50, axis=0)])  # 8 cells x 50 points
+    om = 2 * np.pi * t / np.asarray(Kd) - np.pi
+    A, B = _pair(Nd, om, Jd=6, Kd=Kd, Ld=1024, precision=precision)
+    A.set_binning((16, 16), max_points, "tiled")
+    x = trj.complex_gaussian(Nd + (2,), seed=max_points)
+    y = A.fft(x)
+    assert rel_l2(y, B.fft(x)) <= TOL[precision]
+    xo = B.adj(y)
+    assert rel_l2(A.adj(y), xo) <= TOL[precision]
+    if precision == "single":
+        A.set_scatter("float64")
+        assert rel_l2(A.adj(y), xo) <= TOL[precision]
+
+
 @pytest.mark.parametrize("ndim", [1, 3])
 def test_tiled_1d_3d_small_grids(ndim):
     """Tiles wider than the grid (K < B + J - 1) wrap onto themselves."""
