@@ -39,7 +39,7 @@ TRAFFIC_BYTES = {"spread": 18007664, "interp": 12857829}
 # The same for the 3-D 128^3 / 256^3 kooshball kernels (16^3 bins, fixed-point
 # scatter tile; profiles/nufft3d_128_c64_hbm_traffic.md) and the 24-frame x
 # 32-coil batch (profiles/nufft2d_256_c64_frames_kernel_stats_and_traffic.md).
-TRAFFIC_BYTES_C3 = {"interp": 375.8e6, "spread": 526.2e6}
+TRAFFIC_BYTES_C3 = {"interp": 369.2e6, "spread": 516.4e6}
 TRAFFIC_BYTES_C4 = {"interp": 2468.2e6, "spread": 2102.3e6}  # 64x32 bins, 1,024 points per subproblem
 
 

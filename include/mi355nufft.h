@@ -130,7 +130,7 @@ int64_t mi355nufft_num_points(const mi355nufft_plan *plan);
  *               512; the scatter tile, 8 bytes
  *               per point in fixed point or 16 in float64 (see
  *               mi355nufft_set_scatter), must fit the LDS budget)
- *   max_points  points per subproblem (0: default 384 in 2-D; 2560 in 3-D
+ *   max_points  points per subproblem (0: default 384 in 2-D; 3072 in 3-D
  *               with 16x16x16 bins, else 1024; 512 in 1-D)
  *   method      0 auto (tiled when the tile fits in LDS), 1 global-memory
  *               kernels on unsorted points, 2 tiled (error if it cannot),

@@ -595,11 +595,37 @@ __device__ __forceinline__ void scatter_taps(double* tre, double* tim, const Poi
     }
 }
 
+// A tap's two fixed-point components as one signed 64-bit integer,
+// im * 2^32 + re.  Sums of such values are exact in 64 bits and give back
+// both component sums while each stays within int32 (the tile's scale
+// guarantees it): the low 32 bits hold sum(re), and (v - sum(re)) / 2^32 is
+// sum(im) (fx_unpack).
+__device__ __forceinline__ unsigned long long fx_pack(float vr, float vi) {
+    const long long v = (long long)__float2int_rn(vi) * 4294967296LL + (long long)__float2int_rn(vr);
+    return (unsigned long long)v;
+}
+__device__ __forceinline__ void fx_unpack(long long v, int& re, int& im) {
+    re = (int)(unsigned)(unsigned long long)v;
+    im = (int)((v - (long long)re) / 4294967296LL);
+}
+
 // Fixed-point scatter taps (complex64): the tap values of a sample already
-// scaled to fixed-point units, rounded to the nearest integer and added to a
-// 32-bit integer LDS tile (ds_add_u32).  Same taps, same products and the
+// scaled to fixed-point units, rounded to the nearest integer and added to
+// the tile.  PACK: one tile of packed 64-bit cells (fx_pack, one ds_add_u64
+// per tap); else a plane of int32 real parts (tre) and one of imaginary
+// parts (tim), two ds_add_u32 per tap.  Same taps, same products and the
 // same nesting as scatter_taps.
-template <int NDIM, bool CK, int JMAX, bool FIXED>
+template <bool PACK>
+__device__ __forceinline__ void fx_add(int* tre, int* tim, int i, float vr, float vi) {
+    if constexpr (PACK) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(tre) + i, fx_pack(vr, vi));
+    } else {
+        atomicAdd(tre + i, __float2int_rn(vr));
+        atomicAdd(tim + i, __float2int_rn(vi));
+    }
+}
+
+template <int NDIM, bool CK, int JMAX, bool FIXED, bool PACK>
 __device__ __forceinline__ void scatter_taps_fx(int* tre, int* tim, const PointTaps<float, CK, JMAX>& pt, float2 f,
                                                 const AxisGeom* ax, int T0, int T1) {
     if constexpr (NDIM == 1) {
@@ -608,8 +634,7 @@ __device__ __forceinline__ void scatter_taps_fx(int* tre, int* tim, const PointT
             if (FIXED || j1 < ax[0].J) {
                 float vr, vi;
                 cmul_conj<float, CK>(vr, vi, pt.cr[0][j1], pt.ci[0][j1], f.x, f.y);
-                atomicAdd(tre + pt.l[0] + j1, __float2int_rn(vr));
-                atomicAdd(tim + pt.l[0] + j1, __float2int_rn(vi));
+                fx_add<PACK>(tre, tim, pt.l[0] + j1, vr, vi);
             }
     } else {
         auto rows = [&](float v3r, float v3i, int plane) {
@@ -624,8 +649,7 @@ __device__ __forceinline__ void scatter_taps_fx(int* tre, int* tim, const PointT
                         if (FIXED || j1 < ax[0].J) {
                             float vr, vi;
                             cmul_conj<float, CK>(vr, vi, pt.cr[0][j1], pt.ci[0][j1], v2r, v2i);
-                            atomicAdd(tre + row + j1, __float2int_rn(vr));
-                            atomicAdd(tim + row + j1, __float2int_rn(vi));
+                            fx_add<PACK>(tre, tim, row + j1, vr, vi);
                         }
                 }
         };
@@ -759,10 +783,12 @@ __device__ __forceinline__ cx<T> sample_times_conj_phase(const InterpArgs<T, CK>
 // that each point's coefficients serve two repetitions; more tiles per pass
 // were measured no faster, the LDS atomics dominating, and cost occupancy).
 //
-// FXP (complex64 only): the tile accumulates in 32-bit fixed point.  On
-// gfx950 an LDS float64 atomic add at scattered addresses costs about 21
-// CU-cycles per wave64 instruction and a 32-bit integer one about 7 (a
-// float32 one about 194), and the LDS atomics are most of the scatter's time.
+// FXP (complex64 only): the tile accumulates in 32-bit fixed point; in 3-D
+// a cell's two components are packed in one 64-bit word (fx_pack), so that
+// each tap is one ds_add_u64.  On gfx950 an LDS float64 atomic add at scattered
+// addresses costs about 21 CU-cycles per wave64 instruction, a 64-bit
+// integer one about 11, a 32-bit integer one about 7 (a float32 one about
+// 194), and the LDS atomics are most of the scatter's time.
 // Per subproblem and repetition the taps are scaled by 2^sh, sh = 31 - e with
 // 2^(e-1) <= 1.001 A cmax < 2^e, A = sum over its samples of |Re y| + |Im y|
 // and cmax bounding every coefficient product.  Each tap component is at
@@ -789,6 +815,10 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ float red[4];  // block_sum, at most 256 threads
     using Acc = typename std::conditional<FXP, int, double>::type;
+    // FXP: packed 64-bit cells in 3-D (C3 scatter 0.424 -> 0.411 ms at 2,560
+    // points per subproblem, 0.442 -> 0.403 ms at 3,072); two int32 planes in
+    // 1-D and 2-D, where packing measured 0.3 us slower at Kd 512^2
+    constexpr bool PACK = FXP && NDIM == 3;
     const InterpArgs<T, CK>& ia = a.ia;
     const BinGeom& bg = a.bg;
     // RC tiles, each its real then its imaginary plane
@@ -878,7 +908,7 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
                     if (bad[k]) return;
                     f.x = ldexpf(f.x, sh[k]);
                     f.y = ldexpf(f.y, sh[k]);
-                    scatter_taps_fx<NDIM, CK, JMAX, FIXED>(tre, tim, pt, f, ax, T0, T1);
+                    scatter_taps_fx<NDIM, CK, JMAX, FIXED, PACK>(tre, tim, pt, f, ax, T0, T1);
                 } else {
                     scatter_taps<T, NDIM, CK, JMAX, FIXED>(tre, tim, pt, f, ax, T0, T1);
                 }
@@ -927,7 +957,13 @@ __global__ __launch_bounds__(256) void spread_tiled_kernel(TiledArgs<T, CK> a) {
                 const bool badk = bad[k];
                 auto value = [&](int e, T& vr, T& vi) {  // false: an empty cell
                     if constexpr (FXP) {
-                        const int ir = tre[e], ii = tim[e];
+                        int ir, ii;
+                        if constexpr (PACK) {
+                            fx_unpack(reinterpret_cast<const long long*>(tre)[e], ir, ii);
+                        } else {
+                            ir = tre[e];
+                            ii = tim[e];
+                        }
                         if (badk) {
                             vr = vi = __int_as_float(0x7fc00000);
                             return true;

@@ -699,7 +699,7 @@ namespace mi355host {
 
 // Default bin geometry (autotuned on golden-angle radial and kooshball
 // trajectories): 2-D 16x16 with P = 384 points per subproblem; 3-D 16^3
-// with P = 2560 when the scatter tile is fixed point and no sparse matrix is
+// with P = 3072 when the scatter tile is fixed point and no sparse matrix is
 // set, else 16x8x8 with P = 1024; 1-D 512 with P = 512.  Requested sizes
 // override.
 void default_bins(const mi355nufft_plan* p, int B[3], int& P) {
@@ -719,13 +719,13 @@ void default_bins(const mi355nufft_plan* p, int B[3], int& P) {
     } else if (fxp_eligible(p) && !p->sparse_bins) {
         // the fixed-point scatter tile of 21^3 cells takes 74 KB of LDS:
         // bigger bins cut the halo and the flush (measured at Kd 256^3:
-        // scatter 0.59 -> 0.45 ms, gather unchanged).  P 2560: on the
-        // kooshball pair 0.83 ms at P 2048 and 2304, 0.81 at 2560, 0.82 at
-        // 3072, 0.87-0.89 at 2816, 3584 and 4096
+        // scatter 0.59 -> 0.45 ms, gather unchanged).  P 3072: on the
+        // kooshball, with the packed 64-bit tile cells, pair 0.82 ms at P
+        // 2048, 0.81 at 2560, 0.79 at 3072
         B[0] = 16;
         B[1] = 16;
         B[2] = 16;
-        P = 2560;
+        P = 3072;
     } else {
         B[0] = 16;
         B[1] = 8;
