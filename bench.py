@@ -184,7 +184,7 @@ def measure_atomic_peaks():
     lib = _lib.get_lib()
     stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     out = {}
-    for kind, name in ((0, "lds_f64"), (1, "lds_u32"), (2, "global_f32")):
+    for kind, name in ((0, "lds_f64"), (1, "lds_u32"), (2, "global_f32"), (3, "lds_u64")):
         v = ctypes.c_double(0.0)
         _lib.check(lib.mi355nufft_measure_atomics(kind, ctypes.byref(v), stream))
         out[name] = v.value
@@ -717,8 +717,12 @@ def main():
                                                  "spread": round(TRAFFIC_BYTES_C3["spread"] / (s3 * 1e-3) / 1e9, 1)}
         info3 = r3.A.plan_info()
         acc3 = r3.M * 6 ** 3
-        lds_peak = peaks["lds_u32"] if info3["fixed_point_scatter"] else peaks["lds_f64"]
-        extra["c3_scatter_lds_atomic_frac"] = round(2 * acc3 / (s3 * 1e-3) / lds_peak, 4)
+        # the 3-D fixed-point tile makes one ds_add_u64 per tap (packed
+        # components), the float64 tile two ds_add_f64
+        if info3["fixed_point_scatter"]:
+            extra["c3_scatter_lds_atomic_frac"] = round(acc3 / (s3 * 1e-3) / peaks["lds_u64"], 4)
+        else:
+            extra["c3_scatter_lds_atomic_frac"] = round(2 * acc3 / (s3 * 1e-3) / peaks["lds_f64"], 4)
         extra["c3_scatter_flush_global_atomics"] = flush_atomic_rate(at3["per_launch"], s3, peaks["global_f32"])
         extra["c3_scatter_accumulation"] = "fixed32" if info3["fixed_point_scatter"] else "float64"
         del r3

@@ -357,6 +357,8 @@ int mi355nufft_sparse_apply(mi355nufft_plan *plan, int adjoint, const void *in,
  *           addresses
  *   kind 2  global float32 atomic adds without return, coalesced, over
  *           256 MiB
+ *   kind 3  LDS 64-bit integer atomic adds (ds_add_u64) at scattered
+ *           addresses (the 3-D fixed-point tile's packed cells)
  * ops_per_s receives lane operations per second over the whole device.
  * Allocates and frees its own buffer; waits for the stream.
  */

@@ -1791,7 +1791,7 @@ int mi355nufft_set_scatter(mi355nufft_plan* p, int accumulate) {
 int mi355nufft_measure_atomics(int kind, double* ops_per_s, void* stream) {
     g_err.clear();
     if (!ops_per_s) return fail(MI355NUFFT_EINVAL, "ops_per_s is null");
-    if (kind < 0 || kind > 2) return fail(MI355NUFFT_EINVAL, "kind must be 0, 1 or 2");
+    if (kind < 0 || kind > 3) return fail(MI355NUFFT_EINVAL, "kind must be 0, 1, 2 or 3");
     hipStream_t s = (hipStream_t)stream;
     int dev = 0, ncu = 0;
     HIP_TRY(hipGetDevice(&dev));
@@ -1811,6 +1811,9 @@ int mi355nufft_measure_atomics(int kind, double* ops_per_s, void* stream) {
             hipLaunchKernelGGL(lds_atomic_probe_kernel<double>, dim3(nblk), dim3(256), 0, s, (double*)buf, iters);
         else if (kind == 1)
             hipLaunchKernelGGL(lds_atomic_probe_kernel<unsigned>, dim3(nblk), dim3(256), 0, s, (unsigned*)buf, iters);
+        else if (kind == 3)
+            hipLaunchKernelGGL(lds_atomic_probe_kernel<unsigned long long>, dim3(nblk), dim3(256), 0, s,
+                               (unsigned long long*)buf, iters);
         else
             hipLaunchKernelGGL(global_atomic_probe_kernel, dim3(nblk), dim3(256), 0, s, (float*)buf, gn, greps);
     };

@@ -99,3 +99,25 @@ def test_numpy_results_are_ordinary_arrays():
     assert xa.flags.owndata and xa.flags.f_contiguous and xa.shape == Nd + (3,)
     xa_c = A.adj(np.ascontiguousarray(y1))
     assert rel_l2(xa, xa_c) <= 1e-6
+
+
+def test_measure_atomics_kinds():
+    """mi355nufft_measure_atomics: every kind (LDS f64 / u32 / u64 at
+    scattered addresses, coalesced global f32) returns a finite positive
+    rate; other kinds are rejected on the host before any launch."""
+    import ctypes
+    from mrrt.nufft_amd import _lib
+    dev()
+    lib = _lib.get_lib()
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rates = {}
+    for kind in (0, 1, 2, 3):
+        v = ctypes.c_double(0.0)
+        _lib.check(lib.mi355nufft_measure_atomics(kind, ctypes.byref(v), stream))
+        assert np.isfinite(v.value) and v.value > 0, kind
+        rates[kind] = v.value
+    # the device's LDS does many more lane operations per second than global atomics
+    assert min(rates[0], rates[1], rates[3]) > rates[2]
+    for kind in (-1, 4):
+        v = ctypes.c_double(0.0)
+        assert lib.mi355nufft_measure_atomics(kind, ctypes.byref(v), stream) != 0
