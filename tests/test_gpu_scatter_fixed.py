@@ -172,3 +172,28 @@ def test_points_kept_across_the_scale_sum_vs_oracle(max_points, threads, nreps, 
     yn = y.copy()
     yn[17] = np.nan
     assert np.isnan(A.adj(yn)).any()
+
+
+@pytest.mark.parametrize("method", ["tiled", "tiled_two_phase"])
+def test_packed_3d_tile_vs_oracle(method):
+    """3-D complex64 tiles pack each cell's two fixed-point components into
+    one 64-bit word (im * 2^32 + re).  Negative and positive parts of both
+    components, one repetition 2^40 times another, the atomic flush and the
+    two-phase slots (which unpack every cell), against the oracle and the
+    float64 tile."""
+    Nd, Kd, om = _case(3)
+    A = NufftBase(Nd, om, Jd=6, Kd=Kd, Ld=512, phasing="complex", n_shift=np.asarray(Nd) / 2, device=dev())
+    B = O.NufftOracle(Nd, om, Jd=6, Kd=Kd, Ld=512, phasing="complex", n_shift=np.asarray(Nd) / 2)
+    A.set_binning((16, 16, 16), 512, method)
+    assert A.plan_info()["fixed_point_scatter"]
+    rng = np.random.default_rng(77)
+    y = (rng.standard_normal((om.shape[0], 2)) + 1j * rng.standard_normal((om.shape[0], 2))).astype(np.complex64)
+    y[:, 1] *= np.float32(2.0 ** 40)
+    x = A.adj(y)
+    xo = B.adj(y)
+    for r in range(2):
+        assert rel_l2(x[..., r], xo[..., r]) <= 1e-5, r
+    A.set_scatter("float64")
+    xd = A.adj(y)
+    for r in range(2):
+        assert rel_l2(x[..., r].astype(np.complex128), xd[..., r].astype(np.complex128)) <= 5e-6, r
