@@ -25,9 +25,10 @@ from . import _lib
 __all__ = ["tune", "tune_fft", "candidates", "cache_path"]
 
 # (bin size, points per subproblem, threads per workgroup)
-_C2 = [((b, b), p, t) for b in (16, 32) for p in (64, 128, 256, 512) for t in (64, 128, 256) if p >= t] + \
+_C2 = [((b, b), p, t) for b in (16, 32) for p in (64, 128, 256, 384, 512) for t in (64, 128, 256) if p >= t] + \
     [(b, p, 256) for b in ((64, 32), (32, 64), (64, 64)) for p in (512, 1024)]
-_C3 = [(b, p, t) for b in ((8, 8, 8), (16, 8, 8), (16, 16, 16)) for p in (512, 1024, 2048) for t in (64, 256)]
+_C3 = [(b, p, t) for b in ((8, 8, 8), (16, 8, 8), (16, 16, 16)) for p in (512, 1024, 2048) for t in (64, 256)] + \
+    [((16, 16, 16), p, 256) for p in (2560, 3072, 4096)]
 _C1 = [((b,), p, t) for b in (256, 512, 1024) for p in (256, 512) for t in (64, 256)]
 
 
