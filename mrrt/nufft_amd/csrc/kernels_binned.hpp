@@ -704,7 +704,7 @@ __global__ __launch_bounds__(256) void interp_tiled_kernel(TiledArgs<T, CK> a) {
                 __syncthreads();
                 if constexpr (NDIM >= 2) {
                     // several points per thread (3-D subproblems hold up to
-                    // 2,560, 2-D ones 384): the next point's position, index
+                    // 3,072, 2-D ones 384): the next point's position, index
                     // and phase load while this one's taps are summed
                     // (measured 8% faster at Kd 256^3, where the tile's LDS
                     // bounds occupancy, not the extra registers; 2-D at Kd
