@@ -174,17 +174,21 @@ def test_points_kept_across_the_scale_sum_vs_oracle(max_points, threads, nreps, 
     assert np.isnan(A.adj(yn)).any()
 
 
-@pytest.mark.parametrize("method", ["tiled", "tiled_two_phase"])
-def test_packed_3d_tile_vs_oracle(method):
+@pytest.mark.parametrize("bins,max_points,threads,method", [
+    ((16, 16, 16), 512, 256, "tiled"),             # one repetition per pass
+    ((16, 16, 16), 512, 256, "tiled_two_phase"),   # tile slots: every cell unpacked
+    ((8, 8, 8), 512, 256, "tiled"),                # two 13^3 tiles per pass
+    ((8, 8, 8), 128, 128, "tiled"),                # one point per thread: coefficient cache
+])
+def test_packed_3d_tile_vs_oracle(bins, max_points, threads, method):
     """3-D complex64 tiles pack each cell's two fixed-point components into
     one 64-bit word (im * 2^32 + re).  Negative and positive parts of both
-    components, one repetition 2^40 times another, the atomic flush and the
-    two-phase slots (which unpack every cell), against the oracle and the
-    float64 tile."""
+    components, one repetition 2^40 times another, through each scatter
+    variant, against the oracle and the float64 tile."""
     Nd, Kd, om = _case(3)
     A = NufftBase(Nd, om, Jd=6, Kd=Kd, Ld=512, phasing="complex", n_shift=np.asarray(Nd) / 2, device=dev())
     B = O.NufftOracle(Nd, om, Jd=6, Kd=Kd, Ld=512, phasing="complex", n_shift=np.asarray(Nd) / 2)
-    A.set_binning((16, 16, 16), 512, method)
+    A.set_binning(bins, max_points, method, threads)
     assert A.plan_info()["fixed_point_scatter"]
     rng = np.random.default_rng(77)
     y = (rng.standard_normal((om.shape[0], 2)) + 1j * rng.standard_normal((om.shape[0], 2))).astype(np.complex64)
