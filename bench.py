@@ -29,7 +29,7 @@ sys.path.insert(0, HERE)
 HBM_PEAK_GBS = 8000.0
 
 KERNEL_NAMES = {"interp": "interp_tiled_kernel<float, 2, false, 6, true, false, true>",
-                "spread": "spread_tiled_kernel<float, 2, false, 6, true, true, false>"}
+                "spread": "spread_tiled_kernel<float, 2, false, 6, true, true, false, 1>"}
 
 # HBM bytes per launch of the 2-D 256^2 complex64 kernels, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE passes over this bench (--calibrate-copy), each
